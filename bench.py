@@ -1,0 +1,191 @@
+"""Benchmark: band-H-psi/s of the full FD Hamiltonian at 112^3 (FP64).
+
+Workload (one step = one H psi over all bands): 256 real float64 bands on a
+112^3 periodic grid (h = 0.2 A), 19-point kinetic stencil fused with the
+local potential, projections P = <p|psi> of 64 atoms x 8 projectors,
+P dH, and the projector add -- gpaw.hamiltonian.Hamiltonian.apply.  Inputs
+are seeded synthetic data already resident in HBM.
+
+Prints one JSON line.  `roofline` describes the dominant kernel (the fused
+stencil + potential pass): algorithmic bytes per launch = nbands * 2 * N * 8
+(read psi, write H psi) + N * 8 (potential), divided by its mean duration
+from HIP events on the launch stream.  `cpu_baseline` times the reference's
+own C kernels (oracle/_ref, built from the reference sources) on a bounded
+sample of bands on the host cores.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = ('band-Hψ/s at 112³ (GB/s vs 8 TB/s); RMM-DIIS time to '
+          '1e-8, NN vs MG precond')
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument('--gpus', type=int, default=1)
+    p.add_argument('--steps', type=int, default=20)
+    p.add_argument('--warmup', type=int, default=3)
+    p.add_argument('--nbands', type=int, default=256)
+    p.add_argument('--grid', type=int, default=112)
+    p.add_argument('--natoms', type=int, default=64)
+    p.add_argument('--cpu-bands', type=int, default=32)
+    p.add_argument('--no-cpu', action='store_true')
+    return p.parse_args()
+
+
+def build_workload(N, natoms, nbands):
+    import torch
+    from gpaw_amd import synthetic as syn
+    from gpaw_amd.fd_operators import Laplace
+    from gpaw_amd.grid_descriptor import GridDescriptor
+    from gpaw_amd.hamiltonian import FDHamiltonian
+    from gpaw_amd.lfc import LFC
+    from gpaw_amd.spline import Spline
+    w = syn.full_hpsi_geometry(N, natoms)
+    gd = GridDescriptor((N,) * 3, (w.L,) * 3)
+    pt = LFC(gd, [[Spline.from_data(l, rc, f) for l, rc, f in p]
+                  for p in w.proj])
+    pt.set_positions(w.spos_ac)
+    vt = syn.potential(gd, w.pos_av)
+    kin = Laplace(gd, -0.5, 3)
+    H = FDHamiltonian(gd, kin, vt, pt, w.dH)
+    g = torch.Generator(device='cuda')
+    g.manual_seed(1)
+    psi = torch.randn((nbands, N, N, N), dtype=torch.float64, device='cuda',
+                      generator=g)
+    out = torch.empty_like(psi)
+    return w, gd, pt, vt, kin, H, psi, out
+
+
+def cpu_baseline(w, gd_args, vt, nb_sample):
+    """Reference C path (oracle/_ref) on `nb_sample` bands: Operator.apply,
+    the NumPy potential multiply, LFC.integrate, P dH, LFC.add."""
+    from oracle import refbuild, grid as og, spline as osp
+    from oracle.stencil import laplace_stencil
+    if not refbuild.available():
+        return None
+    N, L = gd_args
+    ogd = og.Grid((N,) * 3, (L,) * 3)
+    c, o = laplace_stencil(ogd.h_cv, -0.5, 3)
+    op = refbuild.Operator(c, o, ogd, True)
+    spl = [[osp.Spline(l, rc, f) for l, rc, f in p] for p in w.proj]
+    lfc = refbuild.LFC(ogd, spl, w.spos_ac)
+    rng = np.random.default_rng(1)
+    psi = rng.standard_normal((nb_sample, N, N, N))
+    out = np.empty_like(psi)
+
+    def step():
+        op.apply(psi, out, None)
+        out.__iadd__(psi * vt)
+        P = lfc.integrate(psi)
+        Q = np.empty_like(P)
+        for a in range(w.natoms):
+            Q[:, 8 * a:8 * a + 8] = P[:, 8 * a:8 * a + 8] @ w.dH[a]
+        lfc.add(out, Q)
+    step()  # warm-up
+    times = []
+    t_end = time.perf_counter() + 20.0
+    while len(times) < 5 or (time.perf_counter() < t_end and len(times) < 9):
+        t0 = time.perf_counter()
+        step()
+        times.append(time.perf_counter() - t0)
+        if time.perf_counter() > t_end + 10:
+            break
+    t = float(np.median(times))
+    threads = int(os.environ.get('OMP_NUM_THREADS', os.cpu_count()))
+    return {'value': nb_sample / t, 'unit': 'band-Hψ/s',
+            'cores': threads, 'kind': 'reference',
+            'sample': ('%d of the 256 bands, same 112^3 grid, 64 atoms; '
+                       'reference c/operators.c + c/lfc.c (OpenMP over '
+                       'bands) + NumPy potential/dH; median of %d runs '
+                       '(%.2f s each)' % (nb_sample, len(times), t))}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print('bench.py measures one GPU: --gpus %d is not supported (the '
+              'path shards by bands; see the design notes)' % args.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+    import torch
+    assert torch.cuda.is_available(), 'needs a GPU'
+    N, natoms, nb = args.grid, args.natoms, args.nbands
+    w, gd, pt, vt, kin, H, psi, out = build_workload(N, natoms, nb)
+    ngpts = N ** 3
+    stream = torch.cuda.current_stream()
+
+    for _ in range(args.warmup):
+        H.apply(psi, out)
+    torch.cuda.synchronize()
+
+    ev = [(torch.cuda.Event(enable_timing=True),
+           torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        ev[s][0].record(stream)
+        H.apply_local(psi, out)          # fused stencil + potential
+        ev[s][1].record(stream)
+        P = pt.integrate_M(psi)
+        pt.add_M(out, H.dH(P))
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    elapsed = t1 - t0
+    ms_step = 1e3 * elapsed / args.steps
+    value = nb * args.steps / elapsed
+    stencil_ms = float(np.mean([a.elapsed_time(b) for a, b in ev]))
+    alg_bytes = nb * 2 * ngpts * 8 + ngpts * 8
+    achieved = alg_bytes / (stencil_ms * 1e-3) / 1e9
+    full_bytes_per_band = 2 * ngpts * 8 + (ngpts * 8 + 8 * pt.nentries *
+                                           8) / nb
+    full_gbs = value * full_bytes_per_band / 1e9
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(w, (N, w.L), vt, args.cpu_bands)
+
+    line = {
+        'metric': METRIC,
+        'value': round(value, 1),
+        'unit': 'band-Hψ/s',
+        'n_gpus': 1,
+        'steps': args.steps,
+        'warmup': args.warmup,
+        'ms_per_step': round(ms_step, 4),
+        'higher_is_better': True,
+        'scaling': 'weak',
+        'vs_baseline': None,
+        'dtype': 'f64',
+        'data': 'synthetic (seeded N(0,1) bands, random atoms/projectors/'
+                'potential)',
+        'config': {'workload': 'full FD H-psi FP64: %d bands, %d^3 periodic '
+                               'grid, h=0.2 A, %d atoms x 8 projectors, '
+                               '19-point stencil' % (nb, N, natoms),
+                   'nbands': nb, 'grid': [N, N, N], 'natoms': natoms,
+                   'parallelism': 'single GPU'},
+        'full_hpsi_GBps': round(full_gbs, 1),
+        'roofline': {'bound': 'hbm', 'achieved': round(achieved, 1),
+                     'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
+                     'frac': round(achieved / HBM_PEAK_GBS, 4),
+                     'traffic': None,
+                     'kernel': 'stencil_cfd_kernel<double,R=3> (T+V fused)',
+                     'kernel_ms': round(stencil_ms, 4)},
+        'cpu_baseline': cpu,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == '__main__':
+    main()

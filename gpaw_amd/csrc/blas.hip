@@ -1,0 +1,219 @@
+// Band-wise BLAS-1 kernels and the multigrid V-cycle on gfx950.
+//
+// multi_axpy follows gpaw/utilities/mblas.py:13-32 (y[x] += a[x] x[x]);
+// multi_dotc gives the per-band integrals of rmmdiis.py:124-130,158-164 with
+// FP64 accumulation in a fixed reduction order (deterministic).
+//
+// The V-cycle computes exactly the 14 steps of the reference
+// (c/fd_preconditioner.c:199-349, gpaw/preconditioner.py:74-93), regrouped
+// so that every scalar update is folded into a stencil or transformer
+// epilogue (L = Laplacian of the level, s = step, R0/R1 restrictions,
+// I2/I1 interpolations):
+//   1. r1 = -R0(R)
+//   2. t1 = 4s L1 r1 - r1                      (= q1 of step 3)
+//   3. r2 = R1(t1)
+//   4. t2 = 32s r2 - 256 s^2 L2 r2             (= d2 after step 7)
+//   5. u1 = I2(t2)                             (= q1 of step 8)
+//   6. t1 = 4s r1 - u1                         (= d1 after step 9)
+//   7. u1 = t1 - 4s (L1 t1 - r1)               (= d1 after step 11)
+//   8. d0' = -I1(u1)                           (step 12)
+//   9. d0 = s (L0 d0' - R) - d0'               (steps 13-14)
+// Nine kernels in place of the reference's twenty array passes.
+#include <algorithm>
+
+#include "common.h"
+#include "internal.h"
+
+namespace gh {
+
+constexpr int kDotChunks = 64;
+
+template <typename V, typename S>
+__global__ void multi_axpy_kernel(const double* __restrict__ a,
+                                  const V* __restrict__ x, V* __restrict__ y,
+                                  int64_t n, int64_t total) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        int64_t band = i / n;
+        y[i] = fma_rs((S)a[band], x[i], y[i]);
+    }
+}
+
+// z = alpha * x + beta * y  (real scalars)
+template <typename V, typename S>
+__global__ void axpbyz_kernel(S alpha, const V* __restrict__ x, S beta,
+                              const V* __restrict__ y, V* __restrict__ z,
+                              int64_t total) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
+         i += (int64_t)gridDim.x * blockDim.x)
+        z[i] = vadd(vscale(x[i], alpha), vscale(y[i], beta));
+}
+
+__device__ __forceinline__ double dotc_term(double a, double b) { return a * b; }
+__device__ __forceinline__ double dotc_term(float a, float b) {
+    return (double)a * (double)b;
+}
+template <typename S>
+__device__ __forceinline__ double dotc_term(cplx<S> a, cplx<S> b) {
+    return (double)a.re * (double)b.re + (double)a.im * (double)b.im;
+}
+
+template <typename V>
+__global__ void __launch_bounds__(256)
+dotc_partial_kernel(const V* __restrict__ a, const V* __restrict__ b,
+                    int64_t n, double* __restrict__ partial) {
+    __shared__ double red[4];
+    const int band = blockIdx.y;
+    const int chunk = blockIdx.x;
+    const V* ab = a + (size_t)band * n;
+    const V* bb = b + (size_t)band * n;
+    const int64_t per = (n + kDotChunks - 1) / kDotChunks;
+    const int64_t i0 = chunk * per;
+    const int64_t i1 = min(n, i0 + per);
+    double s = 0.0;
+    for (int64_t i = i0 + threadIdx.x; i < i1; i += blockDim.x)
+        s += dotc_term(ab[i], bb[i]);
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        partial[band * kDotChunks + chunk] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+__global__ void dotc_final_kernel(const double* __restrict__ partial,
+                                  double scale, double* __restrict__ out) {
+    const int band = blockIdx.x;
+    double s = partial[band * kDotChunks + threadIdx.x];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+    if (threadIdx.x == 0) out[band] = s * scale;
+}
+
+template <typename V, typename S>
+int launch_axpbyz(double alpha, const void* x, double beta, const void* y,
+                  void* z, int64_t total, hipStream_t st) {
+    int64_t nblk = std::min<int64_t>((total + 255) / 256, 8192);
+    hipLaunchKernelGGL((axpbyz_kernel<V, S>), dim3((unsigned)nblk), dim3(256), 0,
+                       st, (S)alpha, (const V*)x, (S)beta, (const V*)y, (V*)z,
+                       total);
+    GH_CHECK_LAUNCH();
+    return GH_OK;
+}
+
+int axpbyz(int dtype, double alpha, const void* x, double beta, const void* y,
+           void* z, int64_t total, hipStream_t st) {
+    switch (dtype) {
+        case GH_F64: return launch_axpbyz<double, double>(alpha, x, beta, y, z, total, st);
+        case GH_C128: return launch_axpbyz<cplx<double>, double>(alpha, x, beta, y, z, total, st);
+        case GH_F32: return launch_axpbyz<float, float>(alpha, x, beta, y, z, total, st);
+        case GH_C64: return launch_axpbyz<cplx<float>, float>(alpha, x, beta, y, z, total, st);
+    }
+    set_error("axpbyz: bad dtype");
+    return GH_ERR_ARG;
+}
+
+}  // namespace gh
+
+using namespace gh;
+
+extern "C" {
+
+int gh_multi_axpy(int dtype, int64_t nbands, int64_t n, const double* a_dev,
+                  const void* x_dev, void* y_dev, void* stream) {
+    GH_REQUIRE(a_dev && x_dev && y_dev && n >= 0 && nbands >= 0, GH_ERR_ARG,
+               "multi_axpy: bad argument");
+    int64_t total = nbands * n;
+    if (total == 0) return GH_OK;
+    hipStream_t st = (hipStream_t)stream;
+    int64_t nblk = std::min<int64_t>((total + 255) / 256, 8192);
+    switch (dtype) {
+        case GH_F64: hipLaunchKernelGGL((multi_axpy_kernel<double, double>), dim3(nblk), dim3(256), 0, st, a_dev, (const double*)x_dev, (double*)y_dev, n, total); break;
+        case GH_C128: hipLaunchKernelGGL((multi_axpy_kernel<cplx<double>, double>), dim3(nblk), dim3(256), 0, st, a_dev, (const cplx<double>*)x_dev, (cplx<double>*)y_dev, n, total); break;
+        case GH_F32: hipLaunchKernelGGL((multi_axpy_kernel<float, float>), dim3(nblk), dim3(256), 0, st, a_dev, (const float*)x_dev, (float*)y_dev, n, total); break;
+        case GH_C64: hipLaunchKernelGGL((multi_axpy_kernel<cplx<float>, float>), dim3(nblk), dim3(256), 0, st, a_dev, (const cplx<float>*)x_dev, (cplx<float>*)y_dev, n, total); break;
+        default: set_error("multi_axpy: bad dtype"); return GH_ERR_ARG;
+    }
+    GH_CHECK_LAUNCH();
+    return GH_OK;
+}
+
+int gh_multi_dotc(int dtype, int64_t nbands, int64_t n, const void* a_dev,
+                  const void* b_dev, double scale, double* out_dev,
+                  double* work_dev, void* stream) {
+    GH_REQUIRE(a_dev && b_dev && out_dev && work_dev && n >= 0 && nbands >= 0,
+               GH_ERR_ARG, "multi_dotc: bad argument");
+    GH_REQUIRE(nbands < 65536, GH_ERR_ARG, "multi_dotc: too many bands per call");
+    if (nbands == 0) return GH_OK;
+    hipStream_t st = (hipStream_t)stream;
+    dim3 grid(kDotChunks, (unsigned)nbands);
+    switch (dtype) {
+        case GH_F64: hipLaunchKernelGGL((dotc_partial_kernel<double>), grid, dim3(256), 0, st, (const double*)a_dev, (const double*)b_dev, n, work_dev); break;
+        case GH_C128: hipLaunchKernelGGL((dotc_partial_kernel<cplx<double>>), grid, dim3(256), 0, st, (const cplx<double>*)a_dev, (const cplx<double>*)b_dev, n, work_dev); break;
+        case GH_F32: hipLaunchKernelGGL((dotc_partial_kernel<float>), grid, dim3(256), 0, st, (const float*)a_dev, (const float*)b_dev, n, work_dev); break;
+        case GH_C64: hipLaunchKernelGGL((dotc_partial_kernel<cplx<float>>), grid, dim3(256), 0, st, (const cplx<float>*)a_dev, (const cplx<float>*)b_dev, n, work_dev); break;
+        default: set_error("multi_dotc: bad dtype"); return GH_ERR_ARG;
+    }
+    GH_CHECK_LAUNCH();
+    hipLaunchKernelGGL(dotc_final_kernel, dim3((unsigned)nbands), dim3(kDotChunks), 0, st,
+                       (const double*)work_dev, scale, out_dev);
+    GH_CHECK_LAUNCH();
+    return GH_OK;
+}
+
+int64_t gh_precond_scratch_bytes(const gh_stencil* kin0, const gh_stencil* kin1,
+                                 const gh_stencil* kin2, int64_t nbands) {
+    if (!kin0 || !kin1 || !kin2) return -1;
+    int64_t n0 = stencil_points(kin0), n1 = stencil_points(kin1),
+            n2 = stencil_points(kin2);
+    return nbands * (n0 + 3 * n1 + 2 * n2) * dtype_size(stencil_dtype(kin0));
+}
+
+int gh_precond_apply(const gh_stencil* kin0, const gh_stencil* kin1,
+                     const gh_stencil* kin2, const gh_transformer* rest0,
+                     const gh_transformer* rest1,
+                     const gh_transformer* intp2,
+                     const gh_transformer* intp1, double step,
+                     const void* R_dev, void* d0_dev, int64_t nbands,
+                     const double* phase_cd, void* scratch_dev, void* stream) {
+    GH_REQUIRE(kin0 && kin1 && kin2 && rest0 && rest1 && intp2 && intp1 &&
+                   R_dev && d0_dev && scratch_dev,
+               GH_ERR_ARG, "precond_apply: null argument");
+    GH_REQUIRE(R_dev != d0_dev, GH_ERR_ARG, "precond_apply: R and d0 must differ");
+    const int dt = stencil_dtype(kin0);
+    GH_REQUIRE(stencil_dtype(kin1) == dt && stencil_dtype(kin2) == dt, GH_ERR_ARG,
+               "precond_apply: dtype mismatch");
+    const int64_t n0 = stencil_points(kin0), n1 = stencil_points(kin1),
+                  n2 = stencil_points(kin2);
+    GH_REQUIRE(transformer_in_points(rest0) == n0 && transformer_out_points(rest0) == n1 &&
+                   transformer_in_points(rest1) == n1 && transformer_out_points(rest1) == n2 &&
+                   transformer_in_points(intp2) == n2 && transformer_out_points(intp2) == n1 &&
+                   transformer_in_points(intp1) == n1 && transformer_out_points(intp1) == n0,
+               GH_ERR_SHAPE, "precond_apply: level sizes do not match");
+    if (nbands == 0) return GH_OK;
+    hipStream_t st = (hipStream_t)stream;
+    const int vs = dtype_size(dt);
+    char* base = (char*)scratch_dev;
+    void* d0p = base;                 base += nbands * n0 * vs;
+    void* r1 = base;                  base += nbands * n1 * vs;
+    void* t1 = base;                  base += nbands * n1 * vs;
+    void* u1 = base;                  base += nbands * n1 * vs;
+    void* r2 = base;                  base += nbands * n2 * vs;
+    void* t2 = base;
+    const double s = step;
+    const double* ph = phase_cd;
+    int rc;
+#define RUN(x) if ((rc = (x)) != GH_OK) return rc
+    RUN(transformer_apply_scaled(rest0, R_dev, r1, nbands, ph, -1.0, st));
+    RUN(stencil_apply_scaled(kin1, r1, t1, nbands, ph, nullptr, nullptr, 0.0, 4 * s, -1.0, st));
+    RUN(transformer_apply_scaled(rest1, t1, r2, nbands, ph, 1.0, st));
+    RUN(stencil_apply_scaled(kin2, r2, t2, nbands, ph, nullptr, nullptr, 0.0,
+                             -256 * s * s, 32 * s, st));
+    RUN(transformer_apply_scaled(intp2, t2, u1, nbands, ph, 1.0, st));
+    RUN(axpbyz(dt, 4 * s, r1, -1.0, u1, t1, nbands * n1, st));
+    RUN(stencil_apply_scaled(kin1, t1, u1, nbands, ph, nullptr, r1, 4 * s, -4 * s, 1.0, st));
+    RUN(transformer_apply_scaled(intp1, u1, d0p, nbands, ph, -1.0, st));
+    RUN(stencil_apply_scaled(kin0, d0p, d0_dev, nbands, ph, nullptr, R_dev, -s, s, -1.0, st));
+#undef RUN
+    return GH_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,558 @@
+// Finite-difference stencils on gfx950: FDOperator.apply as HIP kernels.
+//
+// Semantics follow the reference's Operator.apply (c/operators.c:250-321):
+// a padded copy whose ghosts are periodic (times the Bloch phase) or zero
+// (c/bc.c:146-308), then out = sum_p coef_p * in[g + offset_p]
+// (c/bmgs/fd.c:9-33).  Here no padded copy is made: ghosts are produced on
+// the fly from the interior.
+//
+// Axis-aligned ("central finite difference") stencils -- every Laplace on an
+// orthorhombic cell -- use stencil_cfd_kernel:
+//   * a block owns one band, a tile of `ty` full (y, z) rows and a run of
+//     `lx` x-planes; it marches along x (the slowest axis);
+//   * each thread owns PPT points of the tile and keeps a register queue of
+//     the 2R+1 x-planes through them, so the x-neighbours cost no extra loads;
+//   * the current plane's tile plus R ghost rows above and below lives in
+//     LDS (double buffered, one barrier per plane); y- and z-neighbours are
+//     read from LDS, the z-wrap is an index wrap inside the LDS row;
+//   * each plane of the tile is a contiguous run of (ty + 2R) * n2 values in
+//     HBM, so every load is fully coalesced.
+// Other stencils (non-orthogonal cells) use stencil_generic_kernel, one
+// thread per point reading its neighbours through the cache hierarchy.
+#include <cstdarg>
+#include <cstring>
+#include <vector>
+#include <algorithm>
+
+#include "common.h"
+#include "internal.h"
+
+namespace gh {
+
+static thread_local char g_err[1024] = "";
+
+void set_error(const char* fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+}
+
+constexpr int kMaxR = 6;        // Laplace n <= 6 (fd_operators.py:21-28)
+constexpr int kMaxGeneric = 96; // stencil points of the generic kernel
+
+template <typename S>
+struct CfdCoefs {
+    S c0;
+    S cm[3][kMaxR];  // coefficient of offset -(j+1) along axis c
+    S cp[3][kMaxR];  // coefficient of offset +(j+1) along axis c
+};
+
+template <typename S>
+struct PhaseArgs {
+    S re[3][2];
+    S im[3][2];
+};
+
+struct CfdGeom {
+    int n0, n1, n2;
+    int pbc0, pbc1, pbc2;
+    int ty, lx;     // rows per tile, planes per block
+    int nyt, nxc;   // tiles along y, chunks along x
+};
+
+// Value of in[x, y, z] for an x that may lie in [-n0, 2 n0) (periodic wrap
+// with phase, or zero), y and z inside the grid.
+template <typename V, typename S>
+__device__ __forceinline__ V load_x(const V* __restrict__ inb, int x, int yz,
+                                    const CfdGeom& g, const PhaseArgs<S>& ph) {
+    if (x >= 0 && x < g.n0) return inb[(size_t)x * g.n1 * g.n2 + yz];
+    if (!g.pbc0) return vzero<V>();
+    int d = x < 0 ? 1 : 0;
+    int xx = x < 0 ? x + g.n0 : x - g.n0;
+    V v = inb[(size_t)xx * g.n1 * g.n2 + yz];
+    return mul_phase(v, ph.re[0][d], ph.im[0][d]);
+}
+
+template <typename V, typename S, int R, int PPT, bool HAS_V, bool HAS_SRC>
+__global__ void __launch_bounds__(1024)
+stencil_cfd_kernel(const V* __restrict__ in, V* __restrict__ out,
+                   const S* __restrict__ vt, const V* __restrict__ src,
+                   S beta, CfdCoefs<S> cf, CfdGeom g, PhaseArgs<S> ph) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    V* lds = reinterpret_cast<V*>(smem_raw);
+
+    const int n2 = g.n2;
+    const int tiles_per_band = g.nyt * g.nxc;
+    const int b = xcd_swizzle(blockIdx.x, gridDim.x);
+    const int band = b / tiles_per_band;
+    const int rem = b - band * tiles_per_band;
+    const int xc = rem / g.nyt;
+    const int yt = rem - xc * g.nyt;
+    const int y0 = yt * g.ty;
+    const int tye = min(g.ty, g.n1 - y0);
+    const int x0 = xc * g.lx;
+    const int x1 = min(x0 + g.lx, g.n0);
+    const size_t plane = (size_t)g.n1 * n2;
+    const size_t bstride = plane * g.n0;
+    const V* __restrict__ inb = in + band * bstride;
+    V* __restrict__ outb = out + band * bstride;
+    const V* __restrict__ srcb = HAS_SRC ? src + band * bstride : nullptr;
+    const int tile_pts = tye * n2;
+    const int rows = g.ty + 2 * R;
+    const int bufsz = rows * n2;
+
+    int yl[PPT], zz[PPT];
+    bool act[PPT];
+#pragma unroll
+    for (int k = 0; k < PPT; k++) {
+        int p = threadIdx.x + k * blockDim.x;
+        act[k] = p < tile_pts;
+        int pp = act[k] ? p : 0;
+        yl[k] = pp / n2;
+        zz[k] = pp - yl[k] * n2;
+    }
+
+    // register queue: q[k][j] holds plane (x - R + j) at the thread's point k
+    V q[PPT][2 * R + 1];
+#pragma unroll
+    for (int j = 0; j < 2 * R; j++) {
+#pragma unroll
+        for (int k = 0; k < PPT; k++) {
+            int yz = (y0 + yl[k]) * n2 + zz[k];
+            q[k][j] = act[k] ? load_x(inb, x0 - R + j, yz, g, ph) : vzero<V>();
+        }
+    }
+
+    for (int x = x0; x < x1; x++) {
+#pragma unroll
+        for (int k = 0; k < PPT; k++) {
+            int yz = (y0 + yl[k]) * n2 + zz[k];
+            q[k][2 * R] = act[k] ? load_x(inb, x + R, yz, g, ph) : vzero<V>();
+        }
+        V* buf = lds + (x & 1) * bufsz;
+#pragma unroll
+        for (int k = 0; k < PPT; k++)
+            if (act[k]) buf[(yl[k] + R) * n2 + zz[k]] = q[k][R];
+        // ghost rows above and below the tile (y wraps with phase or zero)
+        const V* __restrict__ pl = inb + (size_t)x * plane;
+        for (int i = threadIdx.x; i < 2 * R * n2; i += blockDim.x) {
+            int hr = i / n2;
+            int z = i - hr * n2;
+            int lrow = hr < R ? hr : hr + tye;
+            int y = y0 - R + lrow;
+            V v;
+            if (y >= 0 && y < g.n1) {
+                v = pl[y * n2 + z];
+            } else if (g.pbc1) {
+                int d = y < 0 ? 1 : 0;
+                int yy = y < 0 ? y + g.n1 : y - g.n1;
+                v = mul_phase(pl[yy * n2 + z], ph.re[1][d], ph.im[1][d]);
+            } else {
+                v = vzero<V>();
+            }
+            buf[lrow * n2 + z] = v;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < PPT; k++) {
+            if (!act[k]) continue;
+            const int row = yl[k] + R;
+            const int z = zz[k];
+            V acc = vscale(q[k][R], cf.c0);
+#pragma unroll
+            for (int j = 0; j < R; j++) {
+                acc = fma_rs(cf.cm[0][j], q[k][R - 1 - j], acc);
+                acc = fma_rs(cf.cp[0][j], q[k][R + 1 + j], acc);
+            }
+#pragma unroll
+            for (int j = 0; j < R; j++) {
+                acc = fma_rs(cf.cm[1][j], buf[(row - 1 - j) * n2 + z], acc);
+                acc = fma_rs(cf.cp[1][j], buf[(row + 1 + j) * n2 + z], acc);
+            }
+            const V* brow = buf + row * n2;
+#pragma unroll
+            for (int j = 0; j < R; j++) {
+                int zm = z - 1 - j;
+                V vm;
+                if (zm >= 0) vm = brow[zm];
+                else if (g.pbc2) vm = mul_phase(brow[zm + n2], ph.re[2][1], ph.im[2][1]);
+                else vm = vzero<V>();
+                int zp = z + 1 + j;
+                V vp;
+                if (zp < n2) vp = brow[zp];
+                else if (g.pbc2) vp = mul_phase(brow[zp - n2], ph.re[2][0], ph.im[2][0]);
+                else vp = vzero<V>();
+                acc = fma_rs(cf.cm[2][j], vm, acc);
+                acc = fma_rs(cf.cp[2][j], vp, acc);
+            }
+            const size_t gi = (size_t)x * plane + (size_t)(y0 + yl[k]) * n2 + z;
+            if (HAS_V) acc = fma_rs(vt[gi], q[k][R], acc);
+            if (HAS_SRC) acc = fma_rs(beta, srcb[gi], acc);
+            outb[gi] = acc;
+        }
+#pragma unroll
+        for (int k = 0; k < PPT; k++)
+#pragma unroll
+            for (int j = 0; j < 2 * R; j++) q[k][j] = q[k][j + 1];
+    }
+}
+
+// ------------------------------------------------------ generic stencil --
+template <typename S>
+struct GenericArgs {
+    int np;
+    int n0, n1, n2;
+    int pbc0, pbc1, pbc2;
+    S coef[kMaxGeneric];
+    signed char off[kMaxGeneric][3];
+};
+
+template <typename V, typename S>
+__device__ __forceinline__ V fetch_wrapped(const V* __restrict__ inb, int x,
+                                           int y, int z,
+                                           const GenericArgs<S>& a,
+                                           const PhaseArgs<S>& ph) {
+    S pre = 1, pim = 0;
+    int c[3] = {x, y, z};
+    const int n[3] = {a.n0, a.n1, a.n2};
+    const int pbc[3] = {a.pbc0, a.pbc1, a.pbc2};
+    for (int ax = 0; ax < 3; ax++) {
+        int d = -1;
+        if (c[ax] < 0) { c[ax] += n[ax]; d = 1; }
+        else if (c[ax] >= n[ax]) { c[ax] -= n[ax]; d = 0; }
+        if (d >= 0) {
+            if (!pbc[ax]) return vzero<V>();
+            S r = pre * ph.re[ax][d] - pim * ph.im[ax][d];
+            S i = pre * ph.im[ax][d] + pim * ph.re[ax][d];
+            pre = r;
+            pim = i;
+        }
+    }
+    V v = inb[((size_t)c[0] * a.n1 + c[1]) * a.n2 + c[2]];
+    return mul_phase(v, pre, pim);
+}
+
+template <typename V, typename S, bool HAS_V, bool HAS_SRC>
+__global__ void __launch_bounds__(256)
+stencil_generic_kernel(const V* __restrict__ in, V* __restrict__ out,
+                       const S* __restrict__ vt, const V* __restrict__ src,
+                       S beta, S gamma, GenericArgs<S> a, PhaseArgs<S> ph,
+                       int64_t total) {
+    const size_t npts = (size_t)a.n0 * a.n1 * a.n2;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        size_t band = t / npts;
+        size_t gi = t - band * npts;
+        int z = gi % a.n2;
+        int y = (gi / a.n2) % a.n1;
+        int x = gi / ((size_t)a.n1 * a.n2);
+        const V* inb = in + band * npts;
+        V acc = vzero<V>();
+        for (int p = 0; p < a.np; p++)
+            acc = fma_rs(a.coef[p],
+                         fetch_wrapped(inb, x + a.off[p][0], y + a.off[p][1],
+                                       z + a.off[p][2], a, ph),
+                         acc);
+        if (gamma != (S)0) acc = fma_rs(gamma, inb[gi], acc);
+        if (HAS_V) acc = fma_rs(vt[gi], inb[gi], acc);
+        if (HAS_SRC) acc = fma_rs(beta, src[t], acc);
+        out[t] = acc;
+    }
+}
+
+}  // namespace gh
+
+// ------------------------------------------------------------- handle --
+// (declared before use by the internal launcher below)
+struct gh_stencil {
+    int dtype;
+    int64_t n[3];
+    int pbc[3];
+    int R;          // max |offset| per axis (padding of the reference)
+    bool cfd;
+    double diag;
+    // cfd representation
+    double c0;
+    double cm[3][gh::kMaxR];
+    double cp[3][gh::kMaxR];
+    // generic representation
+    std::vector<double> coefs;
+    std::vector<int64_t> offs;
+    int ty_req = 0, lx_req = 0;
+};
+
+using namespace gh;
+
+namespace {
+
+template <typename S>
+PhaseArgs<S> phase_args(const double* phase_cd) {
+    Phases p = make_phases(phase_cd);
+    PhaseArgs<S> a;
+    for (int c = 0; c < 3; c++)
+        for (int d = 0; d < 2; d++) {
+            a.re[c][d] = (S)p.re[c][d];
+            a.im[c][d] = (S)p.im[c][d];
+        }
+    return a;
+}
+
+struct TileChoice {
+    int ty, lx, ppt, threads;
+    size_t lds;
+};
+
+TileChoice choose_tile(const gh_stencil* h, int vsize, int ppt) {
+    const int n1 = (int)h->n[1], n2 = (int)h->n[2], n0 = (int)h->n[0];
+    const int R = h->R;
+    const size_t lds_cap = 64 * 1024;
+    TileChoice t{};
+    t.ppt = ppt;
+    int ty = h->ty_req > 0 ? h->ty_req : 0;
+    if (ty == 0) {
+        // largest divisor-friendly row count that fits the LDS and 1024
+        // threads; prefer divisors of n1 to avoid a ragged last tile
+        int best = 1;
+        for (int c = 1; c <= n1; c++) {
+            size_t lds = 2 * (size_t)(c + 2 * R) * n2 * vsize;
+            int thr = (c * n2 + ppt - 1) / ppt;
+            if (lds > lds_cap || thr > 1024) break;
+            if (n1 % c == 0 || c == n1) best = c;
+        }
+        ty = best;
+        // ragged tiles are fine if no divisor >= 8 exists
+        if (ty < 8) {
+            for (int c = n1; c >= 1; c--) {
+                size_t lds = 2 * (size_t)(c + 2 * R) * n2 * vsize;
+                int thr = (c * n2 + ppt - 1) / ppt;
+                if (lds <= lds_cap && thr <= 1024) { ty = c; break; }
+            }
+        }
+    }
+    ty = std::min(ty, n1);
+    t.ty = ty;
+    t.lx = h->lx_req > 0 ? std::min<int>(h->lx_req, n0) : n0;
+    int thr = (ty * n2 + ppt - 1) / ppt;
+    t.threads = ((thr + 63) / 64) * 64;
+    t.lds = 2 * (size_t)(ty + 2 * R) * n2 * vsize;
+    return t;
+}
+
+template <typename V, typename S, int R, int PPT>
+int launch_cfd_r(const gh_stencil* h, const void* in, void* out,
+                 int64_t nbands, const double* phase_cd, const void* vt,
+                 const void* src, double beta, double alpha, double gamma,
+                 hipStream_t st) {
+    TileChoice t = choose_tile(h, sizeof(V), PPT);
+    GH_REQUIRE(t.threads <= 1024 && t.lds <= 64 * 1024, GH_ERR_UNSUPPORTED,
+               "stencil: grid rows of %lld points do not fit one block",
+               (long long)h->n[2]);
+    CfdGeom g;
+    g.n0 = (int)h->n[0]; g.n1 = (int)h->n[1]; g.n2 = (int)h->n[2];
+    g.pbc0 = h->pbc[0]; g.pbc1 = h->pbc[1]; g.pbc2 = h->pbc[2];
+    g.ty = t.ty; g.lx = t.lx;
+    g.nyt = (g.n1 + t.ty - 1) / t.ty;
+    g.nxc = (g.n0 + t.lx - 1) / t.lx;
+    CfdCoefs<S> cf;
+    cf.c0 = (S)(alpha * h->c0 + gamma);
+    for (int c = 0; c < 3; c++)
+        for (int j = 0; j < kMaxR; j++) {
+            cf.cm[c][j] = (S)(alpha * h->cm[c][j]);
+            cf.cp[c][j] = (S)(alpha * h->cp[c][j]);
+        }
+    PhaseArgs<S> ph = phase_args<S>(phase_cd);
+    int64_t nblocks = nbands * (int64_t)g.nyt * g.nxc;
+    GH_REQUIRE(nblocks < (1ll << 31), GH_ERR_SHAPE, "stencil: too many blocks");
+    dim3 grid((unsigned)nblocks), block(t.threads);
+    const S* vtp = (const S*)vt;
+    const V* srcp = (const V*)src;
+    if (vt && src)
+        hipLaunchKernelGGL((stencil_cfd_kernel<V, S, R, PPT, true, true>), grid, block, t.lds, st,
+                           (const V*)in, (V*)out, vtp, srcp, (S)beta, cf, g, ph);
+    else if (vt)
+        hipLaunchKernelGGL((stencil_cfd_kernel<V, S, R, PPT, true, false>), grid, block, t.lds, st,
+                           (const V*)in, (V*)out, vtp, srcp, (S)beta, cf, g, ph);
+    else if (src)
+        hipLaunchKernelGGL((stencil_cfd_kernel<V, S, R, PPT, false, true>), grid, block, t.lds, st,
+                           (const V*)in, (V*)out, vtp, srcp, (S)beta, cf, g, ph);
+    else
+        hipLaunchKernelGGL((stencil_cfd_kernel<V, S, R, PPT, false, false>), grid, block, t.lds, st,
+                           (const V*)in, (V*)out, vtp, srcp, (S)beta, cf, g, ph);
+    GH_CHECK_LAUNCH();
+    return GH_OK;
+}
+
+template <typename V, typename S, int PPT>
+int launch_cfd(const gh_stencil* h, const void* in, void* out, int64_t nb,
+               const double* ph, const void* vt, const void* src, double beta,
+               double al, double ga, hipStream_t st) {
+    switch (h->R) {
+        case 1: return launch_cfd_r<V, S, 1, PPT>(h, in, out, nb, ph, vt, src, beta, al, ga, st);
+        case 2: return launch_cfd_r<V, S, 2, PPT>(h, in, out, nb, ph, vt, src, beta, al, ga, st);
+        case 3: return launch_cfd_r<V, S, 3, PPT>(h, in, out, nb, ph, vt, src, beta, al, ga, st);
+        case 4: return launch_cfd_r<V, S, 4, PPT>(h, in, out, nb, ph, vt, src, beta, al, ga, st);
+        case 5: return launch_cfd_r<V, S, 5, PPT>(h, in, out, nb, ph, vt, src, beta, al, ga, st);
+        case 6: return launch_cfd_r<V, S, 6, PPT>(h, in, out, nb, ph, vt, src, beta, al, ga, st);
+    }
+    set_error("stencil: radius %d unsupported", h->R);
+    return GH_ERR_UNSUPPORTED;
+}
+
+template <typename V, typename S>
+int launch_generic(const gh_stencil* h, const void* in, void* out,
+                   int64_t nbands, const double* phase_cd, const void* vt,
+                   const void* src, double beta, double alpha, double gamma,
+                   hipStream_t st) {
+    GenericArgs<S> a;
+    a.np = (int)h->coefs.size();
+    a.n0 = (int)h->n[0]; a.n1 = (int)h->n[1]; a.n2 = (int)h->n[2];
+    a.pbc0 = h->pbc[0]; a.pbc1 = h->pbc[1]; a.pbc2 = h->pbc[2];
+    for (int p = 0; p < a.np; p++) {
+        a.coef[p] = (S)(alpha * h->coefs[p]);
+        for (int c = 0; c < 3; c++) a.off[p][c] = (signed char)h->offs[p * 3 + c];
+    }
+    PhaseArgs<S> ph = phase_args<S>(phase_cd);
+    int64_t total = nbands * h->n[0] * h->n[1] * h->n[2];
+    int64_t nblk = std::min<int64_t>((total + 255) / 256, 65536);
+    const S* vtp = (const S*)vt;
+    const V* srcp = (const V*)src;
+    if (vt && src)
+        hipLaunchKernelGGL((stencil_generic_kernel<V, S, true, true>), dim3(nblk), dim3(256), 0, st,
+                           (const V*)in, (V*)out, vtp, srcp, (S)beta, (S)gamma, a, ph, total);
+    else if (vt)
+        hipLaunchKernelGGL((stencil_generic_kernel<V, S, true, false>), dim3(nblk), dim3(256), 0, st,
+                           (const V*)in, (V*)out, vtp, srcp, (S)beta, (S)gamma, a, ph, total);
+    else if (src)
+        hipLaunchKernelGGL((stencil_generic_kernel<V, S, false, true>), dim3(nblk), dim3(256), 0, st,
+                           (const V*)in, (V*)out, vtp, srcp, (S)beta, (S)gamma, a, ph, total);
+    else
+        hipLaunchKernelGGL((stencil_generic_kernel<V, S, false, false>), dim3(nblk), dim3(256), 0, st,
+                           (const V*)in, (V*)out, vtp, srcp, (S)beta, (S)gamma, a, ph, total);
+    GH_CHECK_LAUNCH();
+    return GH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* gh_last_error(void) { return gh::g_err; }
+
+const char* gh_version(void) { return "gpaw_amd 0.1 gfx950"; }
+
+int gh_stencil_create(gh_stencil** out, int ncoefs, const double* coefs,
+                      const int64_t* offsets_pc, const int64_t* n_c,
+                      const int32_t* pbc_c, int dtype) {
+    GH_REQUIRE(out && coefs && offsets_pc && n_c && pbc_c, GH_ERR_ARG,
+               "stencil_create: null argument");
+    GH_REQUIRE(ncoefs > 0 && ncoefs <= kMaxGeneric, GH_ERR_ARG,
+               "stencil_create: ncoefs=%d out of range", ncoefs);
+    GH_REQUIRE(dtype >= GH_F64 && dtype <= GH_C64, GH_ERR_ARG,
+               "stencil_create: bad dtype %d", dtype);
+    for (int c = 0; c < 3; c++)
+        GH_REQUIRE(n_c[c] > 0 && n_c[c] < (1 << 20), GH_ERR_SHAPE,
+                   "stencil_create: bad grid size");
+    auto* h = new gh_stencil();
+    h->dtype = dtype;
+    for (int c = 0; c < 3; c++) {
+        h->n[c] = n_c[c];
+        h->pbc[c] = pbc_c[c] ? 1 : 0;
+    }
+    h->coefs.assign(coefs, coefs + ncoefs);
+    h->offs.assign(offsets_pc, offsets_pc + 3 * ncoefs);
+    h->R = 0;
+    h->cfd = true;
+    h->diag = 0.0;
+    h->c0 = 0.0;
+    std::memset(h->cm, 0, sizeof(h->cm));
+    std::memset(h->cp, 0, sizeof(h->cp));
+    for (int p = 0; p < ncoefs; p++) {
+        const int64_t* o = offsets_pc + 3 * p;
+        int nz = (o[0] != 0) + (o[1] != 0) + (o[2] != 0);
+        for (int c = 0; c < 3; c++) {
+            int64_t a = o[c] < 0 ? -o[c] : o[c];
+            if (a > h->R) h->R = (int)a;
+            if (a > n_c[c]) {
+                delete h;
+                set_error("stencil_create: stencil longer than domain");
+                return GH_ERR_SHAPE;
+            }
+        }
+        if (nz == 0) { h->c0 += coefs[p]; h->diag = coefs[p]; }
+        if (nz >= 2) h->cfd = false;
+    }
+    if (h->R > 127) { delete h; set_error("stencil_create: offset too large"); return GH_ERR_UNSUPPORTED; }
+    if (h->cfd && h->R >= 1 && h->R <= kMaxR) {
+        for (int p = 0; p < ncoefs; p++) {
+            const int64_t* o = offsets_pc + 3 * p;
+            for (int c = 0; c < 3; c++) {
+                if (o[c] > 0) h->cp[c][o[c] - 1] += coefs[p];
+                if (o[c] < 0) h->cm[c][-o[c] - 1] += coefs[p];
+            }
+        }
+    } else {
+        h->cfd = false;
+    }
+    *out = h;
+    return GH_OK;
+}
+
+int gh_stencil_destroy(gh_stencil* h) {
+    delete h;
+    return GH_OK;
+}
+
+double gh_stencil_diagonal(const gh_stencil* h) { return h ? h->diag : 0.0; }
+
+int gh_stencil_set_tile(gh_stencil* h, int rows, int planes) {
+    GH_REQUIRE(h && rows >= 0 && planes >= 0, GH_ERR_ARG, "set_tile: bad args");
+    h->ty_req = rows;
+    h->lx_req = planes;
+    return GH_OK;
+}
+
+int gh_stencil_apply(const gh_stencil* h, const void* in_dev, void* out_dev,
+                     int64_t nbands, const double* phase_cd,
+                     const void* vt_dev, const void* src_dev, double beta,
+                     void* stream) {
+    return gh::stencil_apply_scaled(h, in_dev, out_dev, nbands, phase_cd,
+                                    vt_dev, src_dev, beta, 1.0, 0.0,
+                                    (hipStream_t)stream);
+}
+
+}  // extern "C"
+
+namespace gh {
+
+int64_t stencil_points(const gh_stencil* h) { return h->n[0] * h->n[1] * h->n[2]; }
+int stencil_dtype(const gh_stencil* h) { return h->dtype; }
+
+int stencil_apply_scaled(const gh_stencil* h, const void* in_dev, void* out_dev,
+                         int64_t nbands, const double* phase_cd,
+                         const void* vt_dev, const void* src_dev, double beta,
+                         double al, double ga, hipStream_t st) {
+    GH_REQUIRE(h && in_dev && out_dev, GH_ERR_ARG, "stencil_apply: null argument");
+    GH_REQUIRE(in_dev != out_dev, GH_ERR_ARG, "stencil_apply: in and out must differ");
+    GH_REQUIRE(nbands >= 0, GH_ERR_ARG, "stencil_apply: nbands < 0");
+    if (nbands == 0) return GH_OK;
+    if (h->cfd) {
+        switch (h->dtype) {
+            case GH_F64: return launch_cfd<double, double, 2>(h, in_dev, out_dev, nbands, phase_cd, vt_dev, src_dev, beta, al, ga, st);
+            case GH_C128: return launch_cfd<cplx<double>, double, 1>(h, in_dev, out_dev, nbands, phase_cd, vt_dev, src_dev, beta, al, ga, st);
+            case GH_F32: return launch_cfd<float, float, 4>(h, in_dev, out_dev, nbands, phase_cd, vt_dev, src_dev, beta, al, ga, st);
+            case GH_C64: return launch_cfd<cplx<float>, float, 2>(h, in_dev, out_dev, nbands, phase_cd, vt_dev, src_dev, beta, al, ga, st);
+        }
+    } else {
+        switch (h->dtype) {
+            case GH_F64: return launch_generic<double, double>(h, in_dev, out_dev, nbands, phase_cd, vt_dev, src_dev, beta, al, ga, st);
+            case GH_C128: return launch_generic<cplx<double>, double>(h, in_dev, out_dev, nbands, phase_cd, vt_dev, src_dev, beta, al, ga, st);
+            case GH_F32: return launch_generic<float, float>(h, in_dev, out_dev, nbands, phase_cd, vt_dev, src_dev, beta, al, ga, st);
+            case GH_C64: return launch_generic<cplx<float>, float>(h, in_dev, out_dev, nbands, phase_cd, vt_dev, src_dev, beta, al, ga, st);
+        }
+    }
+    set_error("stencil_apply: bad dtype");
+    return GH_ERR_ARG;
+}
+
+}  // namespace gh

@@ -1,0 +1,513 @@
+// Grid restriction and interpolation on gfx950 (Transformer.apply).
+//
+// Semantics follow the reference's Transformer.apply (c/transformers.c:44-
+// 164): the input gets ghost layers (c/bc.c; periodic with Bloch phase or
+// zero, widths from transformers.py:27-47), then three separable 1-D passes
+// along axis 2, 1, 0 (c/bmgs/restrict.c:17-62,136-155;
+// c/bmgs/interpolate.c:9-46,70-100).  Per axis and output index o:
+//   restrict:    c = 2 o + (K-1) - pad_lo
+//                b = 0.5 * (a[c] + sum_j w_j (a[c + 2j+1] + a[c - 2j-1]))
+//   interpolate: q = o + skip_lo, i = q / 2, c = i + nn - 1 - pad_lo
+//                b = a[c]                                  (q even)
+//                b = sum_t w_t (a[c - t] + a[c + 1 + t])   (q odd)
+// with index c in interior coordinates; out-of-range indices are ghosts.
+//
+// One kernel per direction, one pass over HBM: a block owns one band, a
+// tile of full output rows (all z) and a run of output x-planes, and marches
+// along x.  For every input plane it needs, the block stages the input rows
+// of its tile in LDS, filters them along z (LDS -> LDS) and along y (LDS ->
+// registers), and keeps the last 2K-1 (restrict) or K (interpolate) filtered
+// planes in a register queue that the x-filter reads.
+#include <algorithm>
+#include <vector>
+
+#include "common.h"
+#include "internal.h"
+
+namespace gh {
+
+template <typename S>
+struct TPhase {
+    S re[3][2];
+    S im[3][2];
+};
+
+struct TGeom {
+    int n[3];      // input grid
+    int m[3];      // output grid
+    int plo[3];    // ghost width below the interior (pad_cd[:, 0])
+    int skip[3];   // interpolation skip_lo (skip_cd[:, 0])
+    int pbc[3];
+    int tj;        // output rows per tile
+    int lx;        // output planes per block
+    int nyt, nxc;
+    int nry;       // input rows staged per tile (upper bound)
+};
+
+template <int K> struct W;
+template <> struct W<2> { static constexpr double w[1] = {0.5}; };
+template <> struct W<4> { static constexpr double w[2] = {0.5625, -0.0625}; };
+template <> struct W<6> { static constexpr double w[3] = {0.58593750, -0.09765625, 0.01171875}; };
+template <> struct W<8> { static constexpr double w[4] = {0.59814453125, -0.11962890625, 0.02392578125, -0.00244140625}; };
+
+// Wrap an interior index along axis ax (one period at most); returns false
+// for a zero ghost, else sets the phase factor.
+template <typename S>
+__device__ __forceinline__ bool wrap_idx(int& i, int n, int pbc, int ax,
+                                         const TPhase<S>& ph, S& pre, S& pim) {
+    if (i >= 0 && i < n) return true;
+    if (!pbc) return false;
+    int d = i < 0 ? 1 : 0;
+    i += i < 0 ? n : -n;
+    S r = pre * ph.re[ax][d] - pim * ph.im[ax][d];
+    S m = pre * ph.im[ax][d] + pim * ph.re[ax][d];
+    pre = r;
+    pim = m;
+    return true;
+}
+
+// Stage input rows [iy0, iy0 + nrows) of input plane ix (full z) into A.
+template <typename V, typename S>
+__device__ __forceinline__ void stage_rows(const V* __restrict__ inb, int ix,
+                                           int iy0, int nrows, V* A,
+                                           const TGeom& g,
+                                           const TPhase<S>& ph) {
+    const int n2 = g.n[2];
+    S xre = 1, xim = 0;
+    int x = ix;
+    bool xok = wrap_idx(x, g.n[0], g.pbc[0], 0, ph, xre, xim);
+    const V* pl = inb + (size_t)x * g.n[1] * n2;
+    for (int i = threadIdx.x; i < nrows * n2; i += blockDim.x) {
+        int r = i / n2;
+        int z = i - r * n2;
+        int y = iy0 + r;
+        S pre = xre, pim = xim;
+        bool ok = xok && wrap_idx(y, g.n[1], g.pbc[1], 1, ph, pre, pim);
+        A[i] = ok ? mul_phase(pl[y * n2 + z], pre, pim) : vzero<V>();
+    }
+}
+
+// Value of staged row r at interior z index (wrapping along z).
+template <typename V, typename S>
+__device__ __forceinline__ V row_at(const V* row, int z, const TGeom& g,
+                                    const TPhase<S>& ph) {
+    S pre = 1, pim = 0;
+    if (!wrap_idx(z, g.n[2], g.pbc[2], 2, ph, pre, pim)) return vzero<V>();
+    return mul_phase(row[z], pre, pim);
+}
+
+template <typename V, typename S, int K>
+__device__ __forceinline__ V restrict_tap(const V* row, int c, const TGeom& g,
+                                          const TPhase<S>& ph) {
+    V acc = row_at(row, c, g, ph);
+#pragma unroll
+    for (int j = 0; j < K / 2; j++) {
+        int o = 2 * j + 1;
+        acc = fma_rs((S)W<K>::w[j],
+                     vadd(row_at(row, c + o, g, ph), row_at(row, c - o, g, ph)),
+                     acc);
+    }
+    return vscale(acc, (S)0.5);
+}
+
+// Restriction: output tile rows [J0, J0 + tj), output planes [I0, I1).
+template <typename V, typename S, int K, int PPT>
+__global__ void __launch_bounds__(1024)
+restrict_kernel(const V* __restrict__ in, V* __restrict__ out, TGeom g,
+                TPhase<S> ph, S alpha) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int n2 = g.n[2], m2 = g.m[2];
+    V* A = reinterpret_cast<V*>(smem_raw);   // nry x n2
+    V* B = A + g.nry * n2;                   // nry x m2
+    const int per_band = g.nyt * g.nxc;
+    const int b = xcd_swizzle(blockIdx.x, gridDim.x);
+    const int band = b / per_band;
+    const int rem = b - band * per_band;
+    const int xc = rem / g.nyt;
+    const int yt = rem - xc * g.nyt;
+    const int J0 = yt * g.tj;
+    const int tje = min(g.tj, g.m[1] - J0);
+    const int I0 = xc * g.lx;
+    const int I1 = min(I0 + g.lx, g.m[0]);
+    const V* inb = in + (size_t)band * g.n[0] * g.n[1] * n2;
+    V* outb = out + (size_t)band * g.m[0] * g.m[1] * m2;
+    // input rows: centre 2J + K-1 - plo, taps +-(K-1)
+    const int iy0 = 2 * J0 - g.plo[1];
+    const int nrows = 2 * tje + 2 * K - 3;
+    const int npts = tje * m2;
+
+    int jl[PPT], kz[PPT];
+    bool act[PPT];
+#pragma unroll
+    for (int k = 0; k < PPT; k++) {
+        int p = threadIdx.x + k * blockDim.x;
+        act[k] = p < npts;
+        int pp = act[k] ? p : 0;
+        jl[k] = pp / m2;
+        kz[k] = pp - jl[k] * m2;
+    }
+    constexpr int QL = 2 * K - 1;
+    V q[PPT][QL];
+    const int base = 2 * I0 - g.plo[0];  // first input plane of output I0
+
+    auto process_plane = [&](int ix, int slot) {
+        stage_rows<V, S>(inb, ix, iy0, nrows, A, g, ph);
+        __syncthreads();
+        // z-filter every staged row
+        for (int i = threadIdx.x; i < nrows * m2; i += blockDim.x) {
+            int r = i / m2;
+            int kk = i - r * m2;
+            int c = 2 * kk + (K - 1) - g.plo[2];
+            B[r * m2 + kk] = restrict_tap<V, S, K>(A + r * n2, c, g, ph);
+        }
+        __syncthreads();
+        // y-filter (rows are already ghost-filled)
+#pragma unroll
+        for (int k = 0; k < PPT; k++) {
+            if (!act[k]) continue;
+            int c = 2 * jl[k] + (K - 1);
+            V acc = B[c * m2 + kz[k]];
+#pragma unroll
+            for (int j = 0; j < K / 2; j++) {
+                int o = 2 * j + 1;
+                acc = fma_rs((S)W<K>::w[j],
+                             vadd(B[(c + o) * m2 + kz[k]], B[(c - o) * m2 + kz[k]]),
+                             acc);
+            }
+            q[k][slot] = vscale(acc, (S)0.5);
+        }
+    };
+
+    // prime the first 2K-3 planes
+#pragma unroll
+    for (int s = 0; s < QL - 2; s++) process_plane(base + s, s);
+    for (int I = I0; I < I1; I++) {
+        int p0 = base + 2 * (I - I0) + QL - 2;
+        process_plane(p0, QL - 2);
+        process_plane(p0 + 1, QL - 1);
+#pragma unroll
+        for (int k = 0; k < PPT; k++) {
+            if (!act[k]) continue;
+            const int c = K - 1;
+            V acc = q[k][c];
+#pragma unroll
+            for (int j = 0; j < K / 2; j++) {
+                int o = 2 * j + 1;
+                acc = fma_rs((S)W<K>::w[j], vadd(q[k][c + o], q[k][c - o]), acc);
+            }
+            outb[((size_t)I * g.m[1] + J0 + jl[k]) * m2 + kz[k]] =
+                vscale(vscale(acc, (S)0.5), alpha);
+        }
+#pragma unroll
+        for (int k = 0; k < PPT; k++)
+#pragma unroll
+            for (int s = 0; s < QL - 2; s++) q[k][s] = q[k][s + 2];
+    }
+}
+
+template <typename V, typename S, int K>
+__device__ __forceinline__ V interp_tap(const V* row, int o, int skip, int plo,
+                                        const TGeom& g, const TPhase<S>& ph) {
+    constexpr int nn = K / 2;
+    int qq = o + skip;
+    int c = (qq >> 1) + nn - 1 - plo;
+    if ((qq & 1) == 0) return row_at(row, c, g, ph);
+    V acc = vzero<V>();
+#pragma unroll
+    for (int t = 0; t < nn; t++)
+        acc = fma_rs((S)W<K>::w[t],
+                     vadd(row_at(row, c - t, g, ph), row_at(row, c + 1 + t, g, ph)),
+                     acc);
+    return acc;
+}
+
+// Interpolation: output (fine) rows [J0, J0 + tj), fine planes [O0, O1).
+template <typename V, typename S, int K, int PPT>
+__global__ void __launch_bounds__(1024)
+interpolate_kernel(const V* __restrict__ in, V* __restrict__ out, TGeom g,
+                   TPhase<S> ph, S alpha) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    constexpr int nn = K / 2;
+    const int n2 = g.n[2], m2 = g.m[2];
+    V* A = reinterpret_cast<V*>(smem_raw);   // nry x n2
+    V* B = A + g.nry * n2;                   // nry x m2
+    const int per_band = g.nyt * g.nxc;
+    const int b = xcd_swizzle(blockIdx.x, gridDim.x);
+    const int band = b / per_band;
+    const int rem = b - band * per_band;
+    const int xc = rem / g.nyt;
+    const int yt = rem - xc * g.nyt;
+    const int J0 = yt * g.tj;
+    const int tje = min(g.tj, g.m[1] - J0);
+    const int O0 = xc * g.lx;
+    const int O1 = min(O0 + g.lx, g.m[0]);
+    const V* inb = in + (size_t)band * g.n[0] * g.n[1] * n2;
+    V* outb = out + (size_t)band * g.m[0] * g.m[1] * m2;
+    // coarse rows needed by fine rows J0 .. J0 + tje - 1
+    const int cy0 = ((J0 + g.skip[1]) >> 1) - g.plo[1];
+    const int cy1 = ((J0 + tje - 1 + g.skip[1]) >> 1) + 2 * nn - 1 - g.plo[1];
+    const int nrows = cy1 - cy0 + 1;
+    const int npts = tje * m2;
+
+    int jl[PPT], kz[PPT];
+    bool act[PPT];
+#pragma unroll
+    for (int k = 0; k < PPT; k++) {
+        int p = threadIdx.x + k * blockDim.x;
+        act[k] = p < npts;
+        int pp = act[k] ? p : 0;
+        jl[k] = pp / m2;
+        kz[k] = pp - jl[k] * m2;
+    }
+    V q[PPT][K];  // coarse planes c - K + 1 .. c after the y/z filters
+
+    auto process_plane = [&](int ix, int slot) {
+        stage_rows<V, S>(inb, ix, cy0, nrows, A, g, ph);
+        __syncthreads();
+        for (int i = threadIdx.x; i < nrows * m2; i += blockDim.x) {
+            int r = i / m2;
+            int kk = i - r * m2;
+            B[r * m2 + kk] = interp_tap<V, S, K>(A + r * n2, kk, g.skip[2],
+                                                 g.plo[2], g, ph);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < PPT; k++) {
+            if (!act[k]) continue;
+            int J = J0 + jl[k];
+            int qq = J + g.skip[1];
+            int c = (qq >> 1) + nn - 1 - g.plo[1] - cy0;  // local row
+            V v;
+            if ((qq & 1) == 0) {
+                v = B[c * m2 + kz[k]];
+            } else {
+                v = vzero<V>();
+#pragma unroll
+                for (int t = 0; t < nn; t++)
+                    v = fma_rs((S)W<K>::w[t],
+                               vadd(B[(c - t) * m2 + kz[k]], B[(c + 1 + t) * m2 + kz[k]]),
+                               v);
+            }
+            q[k][slot] = v;
+        }
+    };
+
+    // fine plane o -> coarse centre c(o) = ((o + skip) >> 1) + nn - 1 - plo;
+    // it needs coarse planes c - nn + 1 .. c + nn.
+    const int i_first = (O0 + g.skip[0]) >> 1;
+    const int i_last = (O1 - 1 + g.skip[0]) >> 1;
+    const int cbase = nn - 1 - g.plo[0];
+#pragma unroll
+    for (int s = 0; s < K - 1; s++) process_plane(i_first + cbase - nn + 1 + s, s);
+    for (int i = i_first; i <= i_last; i++) {
+        process_plane(i + cbase + nn, K - 1);
+        // queue slot s holds coarse plane (c - nn + 1 + s), c = i + cbase
+        const int oe = 2 * i - g.skip[0];
+#pragma unroll
+        for (int par = 0; par < 2; par++) {
+            int o = oe + par;
+            if (o < O0 || o >= O1) continue;
+#pragma unroll
+            for (int k = 0; k < PPT; k++) {
+                if (!act[k]) continue;
+                V v;
+                if (par == 0) {
+                    v = q[k][nn - 1];
+                } else {
+                    v = vzero<V>();
+#pragma unroll
+                    for (int t = 0; t < nn; t++)
+                        v = fma_rs((S)W<K>::w[t], vadd(q[k][nn - 1 - t], q[k][nn + t]), v);
+                }
+                outb[((size_t)o * g.m[1] + J0 + jl[k]) * m2 + kz[k]] = vscale(v, alpha);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < PPT; k++)
+#pragma unroll
+            for (int s = 0; s < K - 1; s++) q[k][s] = q[k][s + 1];
+    }
+}
+
+}  // namespace gh
+
+struct gh_transformer {
+    int dtype;
+    int nn;
+    bool interp;
+    gh::TGeom g;
+};
+
+using namespace gh;
+
+namespace {
+
+template <typename S>
+TPhase<S> tphase(const double* phase_cd) {
+    Phases p = make_phases(phase_cd);
+    TPhase<S> a;
+    for (int c = 0; c < 3; c++)
+        for (int d = 0; d < 2; d++) {
+            a.re[c][d] = (S)p.re[c][d];
+            a.im[c][d] = (S)p.im[c][d];
+        }
+    return a;
+}
+
+constexpr int kPPT = 2;
+
+// Rows of the output tile: as many as fit 1024 threads and 64 KB LDS.
+int choose_rows(const gh_transformer* h, int vsize, size_t* lds_out,
+                int* nry_out) {
+    const TGeom& g = h->g;
+    const int K = 2 * h->nn;
+    int best = 0;
+    size_t best_lds = 0;
+    int best_nry = 0;
+    for (int tj = 1; tj <= g.m[1]; tj++) {
+        int nry = h->interp ? (tj + 1) / 2 + 2 * h->nn + 1 : 2 * tj + 2 * K - 3;
+        size_t lds = (size_t)nry * (g.n[2] + g.m[2]) * vsize;
+        int thr = (tj * g.m[2] + kPPT - 1) / kPPT;
+        if (lds > 64 * 1024 || thr > 1024) break;
+        if (g.m[1] % tj == 0 || tj == g.m[1] || best == 0) {
+            best = tj;
+            best_lds = lds;
+            best_nry = nry;
+        }
+    }
+    *lds_out = best_lds;
+    *nry_out = best_nry;
+    return best;
+}
+
+template <typename V, typename S, int K>
+int launch_t(const gh_transformer* h, const void* in, void* out,
+             int64_t nbands, const double* phase_cd, double alpha,
+             hipStream_t st) {
+    TGeom g = h->g;
+    size_t lds;
+    int nry;
+    int tj = choose_rows(h, sizeof(V), &lds, &nry);
+    GH_REQUIRE(tj > 0, GH_ERR_UNSUPPORTED,
+               "transformer: grid rows too long for one block");
+    g.tj = tj;
+    g.nry = nry;
+    g.lx = g.m[0];
+    g.nyt = (g.m[1] + tj - 1) / tj;
+    g.nxc = 1;
+    // split x so that there are enough blocks to fill the GPU
+    while ((int64_t)nbands * g.nyt * g.nxc < 2048 && g.lx > 8) {
+        g.lx = (g.lx + 1) / 2;
+        g.nxc = (g.m[0] + g.lx - 1) / g.lx;
+    }
+    int thr = (tj * g.m[2] + kPPT - 1) / kPPT;
+    thr = ((thr + 63) / 64) * 64;
+    int64_t nblocks = nbands * (int64_t)g.nyt * g.nxc;
+    GH_REQUIRE(nblocks < (1ll << 31), GH_ERR_SHAPE, "transformer: too many blocks");
+    TPhase<S> ph = tphase<S>(phase_cd);
+    if (h->interp)
+        hipLaunchKernelGGL((interpolate_kernel<V, S, K, kPPT>), dim3((unsigned)nblocks),
+                           dim3(thr), lds, st, (const V*)in, (V*)out, g, ph, (S)alpha);
+    else
+        hipLaunchKernelGGL((restrict_kernel<V, S, K, kPPT>), dim3((unsigned)nblocks),
+                           dim3(thr), lds, st, (const V*)in, (V*)out, g, ph, (S)alpha);
+    GH_CHECK_LAUNCH();
+    return GH_OK;
+}
+
+template <typename V, typename S>
+int launch_k(const gh_transformer* h, const void* in, void* out, int64_t nb,
+             const double* ph, double al, hipStream_t st) {
+    switch (h->nn) {
+        case 1: return launch_t<V, S, 2>(h, in, out, nb, ph, al, st);
+        case 2: return launch_t<V, S, 4>(h, in, out, nb, ph, al, st);
+        case 3: return launch_t<V, S, 6>(h, in, out, nb, ph, al, st);
+        case 4: return launch_t<V, S, 8>(h, in, out, nb, ph, al, st);
+    }
+    set_error("transformer: nn=%d unsupported", h->nn);
+    return GH_ERR_UNSUPPORTED;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gh_transformer_create(gh_transformer** out, const int64_t* nin_c,
+                          const int64_t* nout_c, int nn,
+                          const int64_t* pad_cd, const int64_t* skip_cd,
+                          const int32_t* pbc_c, int interpolate, int dtype) {
+    GH_REQUIRE(out && nin_c && nout_c && pad_cd && skip_cd && pbc_c, GH_ERR_ARG,
+               "transformer_create: null argument");
+    GH_REQUIRE(nn >= 1 && nn <= 4, GH_ERR_ARG, "transformer_create: nn=%d", nn);
+    GH_REQUIRE(dtype >= GH_F64 && dtype <= GH_C64, GH_ERR_ARG,
+               "transformer_create: bad dtype");
+    auto* h = new gh_transformer();
+    h->dtype = dtype;
+    h->nn = nn;
+    h->interp = interpolate != 0;
+    for (int c = 0; c < 3; c++) {
+        h->g.n[c] = (int)nin_c[c];
+        h->g.m[c] = (int)nout_c[c];
+        h->g.plo[c] = (int)pad_cd[2 * c];
+        h->g.skip[c] = (int)skip_cd[2 * c];
+        h->g.pbc[c] = pbc_c[c] ? 1 : 0;
+        const int64_t lo = pad_cd[2 * c], hi = pad_cd[2 * c + 1];
+        bool ok = lo >= 0 && hi >= 0 && nin_c[c] > 0 && nout_c[c] > 0 &&
+                  (!pbc_c[c] || (lo <= nin_c[c] && hi <= nin_c[c]));
+        if (ok && !h->interp)
+            ok = nout_c[c] == (nin_c[c] + lo + hi - 4 * nn + 3) / 2;
+        if (ok && h->interp)
+            ok = nout_c[c] == 2 * (nin_c[c] + lo + hi - 2 * nn + 1 + skip_cd[2 * c + 1])
+                                  - skip_cd[2 * c] - skip_cd[2 * c + 1];
+        if (!ok) {
+            delete h;
+            set_error("transformer_create: inconsistent sizes/pads on axis %d", c);
+            return GH_ERR_SHAPE;
+        }
+    }
+    *out = h;
+    return GH_OK;
+}
+
+int gh_transformer_destroy(gh_transformer* h) {
+    delete h;
+    return GH_OK;
+}
+
+int gh_transformer_apply(const gh_transformer* h, const void* in_dev,
+                         void* out_dev, int64_t nbands,
+                         const double* phase_cd, void* stream) {
+    return gh::transformer_apply_scaled(h, in_dev, out_dev, nbands, phase_cd,
+                                        1.0, (hipStream_t)stream);
+}
+
+}  // extern "C"
+
+namespace gh {
+
+int64_t transformer_in_points(const gh_transformer* h) {
+    return (int64_t)h->g.n[0] * h->g.n[1] * h->g.n[2];
+}
+int64_t transformer_out_points(const gh_transformer* h) {
+    return (int64_t)h->g.m[0] * h->g.m[1] * h->g.m[2];
+}
+
+int transformer_apply_scaled(const gh_transformer* h, const void* in_dev,
+                             void* out_dev, int64_t nbands,
+                             const double* phase_cd, double al,
+                             hipStream_t st) {
+    GH_REQUIRE(h && in_dev && out_dev, GH_ERR_ARG, "transformer_apply: null argument");
+    GH_REQUIRE(in_dev != out_dev, GH_ERR_ARG, "transformer_apply: in and out must differ");
+    GH_REQUIRE(nbands >= 0, GH_ERR_ARG, "transformer_apply: nbands < 0");
+    if (nbands == 0) return GH_OK;
+    switch (h->dtype) {
+        case GH_F64: return launch_k<double, double>(h, in_dev, out_dev, nbands, phase_cd, al, st);
+        case GH_C128: return launch_k<cplx<double>, double>(h, in_dev, out_dev, nbands, phase_cd, al, st);
+        case GH_F32: return launch_k<float, float>(h, in_dev, out_dev, nbands, phase_cd, al, st);
+        case GH_C64: return launch_k<cplx<float>, float>(h, in_dev, out_dev, nbands, phase_cd, al, st);
+    }
+    set_error("transformer_apply: bad dtype");
+    return GH_ERR_ARG;
+}
+
+}  // namespace gh

@@ -1,0 +1,165 @@
+"""Finite-difference operators (drop-in for gpaw.fd_operators).
+
+FDOperator / Laplace / GUCLaplace keep the reference's constructor and
+method signatures (gpaw/fd_operators.py:36-141,171-232).  The stencil runs
+in libgpawhip (gh_stencil_*); coefficients are derived on the host exactly as
+GUCLaplace does (least squares over the 13 shortest lattice directions,
+Chelikowsky expansion coefficients).
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._arrays import (Staged, dtype_code, real_dtype, require_gpu,
+                      stream_ptr, torch_dtype)
+
+# Expansion coefficients of the finite-difference Laplacian, J. R.
+# Chelikowsky et al., Phys. Rev. B 50, 11355 (1994) -- the table of
+# fd_operators.py:21-28 (a published table; kept identical).
+laplace = [[0],
+           [-2, 1],
+           [-5 / 2, 4 / 3, -1 / 12],
+           [-49 / 18, 3 / 2, -3 / 20, 1 / 90],
+           [-205 / 72, 8 / 5, -1 / 5, 8 / 315, -1 / 560],
+           [-5269 / 1800, 5 / 3, -5 / 21, 5 / 126, -5 / 1008, 1 / 3150],
+           [-5369 / 1800, 12 / 7, -15 / 56, 10 / 189, -1 / 112, 2 / 1925,
+            -1 / 16632]]
+
+
+class FDOperator:
+    """Stencil operator out = sum_p coef_p in[g + offset_p] with periodic
+    (Bloch) or zero boundaries (fd_operators.py:36-141)."""
+
+    def __init__(self, coef_p, offset_pc, gd, dtype=float, description=None,
+                 xp=None):
+        offset_pc = np.asarray(offset_pc, int).reshape(-1, 3)
+        coef_p = np.ascontiguousarray(coef_p, float)
+        assert coef_p.ndim == 1 and len(coef_p) == len(offset_pc)
+        mp = np.abs(offset_pc).max()
+        assert (mp <= gd.n_c).all(), 'Stencil longer than domain'
+        self.cfd = all(sum(o != 0 for o in off) < 2 for off in offset_pc)
+        self.mp = mp
+        self.gd = gd
+        self.dtype = dtype
+        self.tdtype = torch_dtype(dtype)
+        self.shape = tuple(int(n) for n in gd.n_c)
+        self.npoints = len(coef_p)
+        self.coef_p = coef_p
+        self.offset_pc = offset_pc
+        M_c = gd.n_c + 2 * mp
+        self.offset_p = np.dot(offset_pc, [M_c[1] * M_c[2], M_c[2], 1])
+        self.xp = xp
+        L = _lib.lib()
+        h = ctypes.c_void_p()
+        self._keep = [_lib.f64(coef_p), _lib.i64(offset_pc.ravel()),
+                      _lib.i64(gd.n_c), _lib.i32(gd.pbc_c.astype(np.int32))]
+        _lib.check(L.gh_stencil_create(
+            ctypes.byref(h), len(coef_p), self._keep[0][1], self._keep[1][1],
+            self._keep[2][1], self._keep[3][1], dtype_code(dtype)))
+        self.handle = h
+        self._lib = L
+        if description is None:
+            description = '%d point finite-difference stencil' % self.npoints
+        self.description = description
+
+    def __del__(self):
+        h = getattr(self, 'handle', None)
+        if h is not None and h.value:
+            self._lib.gh_stencil_destroy(h)
+            self.handle = None
+
+    def __str__(self):
+        return '<' + self.description + '>'
+
+    def apply(self, in_xg, out_xg, phase_cd=None, vt_g=None, src_xg=None,
+              beta=1.0):
+        """out = L in  [+ vt * in] [+ beta * src]  (fd_operators.py:114).
+
+        vt_g (optional) fuses the local potential of
+        Hamiltonian.apply_local_potential (hamiltonian.py:437-443)."""
+        assert tuple(in_xg.shape) == tuple(out_xg.shape)
+        assert tuple(in_xg.shape[-3:]) == self.shape
+        nb = int(np.prod(in_xg.shape[:-3], dtype=np.int64))
+        ph, php = _lib.phase_arg(phase_cd)
+        with Staged(in_xg, dtype=self.dtype) as a, \
+                Staged(out_xg, writeback=True, dtype=self.dtype) as b:
+            vt = src = None
+            if vt_g is not None:
+                vt = Staged(vt_g, dtype=real_dtype(self.dtype)).t
+                assert tuple(vt.shape) == self.shape
+            if src_xg is not None:
+                src = Staged(src_xg, dtype=self.dtype).t
+                assert src.shape == a.shape
+            _lib.check(self._lib.gh_stencil_apply(
+                self.handle, a.data_ptr(), b.data_ptr(), nb, php,
+                vt.data_ptr() if vt is not None else None,
+                src.data_ptr() if src is not None else None, float(beta),
+                stream_ptr()))
+
+    def __call__(self, in_xR, out_xR=None):
+        if out_xR is None:
+            out_xR = in_xR.new_empty(in_xR.shape) if hasattr(
+                in_xR, 'new_empty') else np.empty_like(in_xR)
+        self.apply(in_xR, out_xR)
+        return out_xR
+
+    def relax(self, relax_method, f_g, s_g, n, w=None):
+        raise NotImplementedError(
+            'Gauss-Seidel/Jacobi relaxation belongs to the Poisson solver, '
+            'which this package does not provide')
+
+    def get_diagonal_element(self):
+        return self._lib.gh_stencil_diagonal(self.handle)
+
+    def set_tile(self, rows=0, planes=0):
+        """Tile shape of the stencil kernel (0 = automatic)."""
+        _lib.check(self._lib.gh_stencil_set_tile(self.handle, int(rows),
+                                                 int(planes)))
+
+
+def Laplace(gd, scale=1.0, n=1, dtype=float, xp=None):
+    if n == 9:
+        raise NotImplementedError('FFT Laplacian (n=9) is not provided')
+    return GUCLaplace(gd, scale, n, dtype, xp=xp)
+
+
+def guc_laplace_stencil(h_cv, scale=1.0, n=1):
+    """Coefficients and offsets of GUCLaplace (fd_operators.py:178-232)."""
+    M_ic = np.indices((3, 3, 3)).reshape((3, -1)).T[-13:] - 1
+    u_cv = h_cv / (h_cv ** 2).sum(1)[:, None] ** 0.5
+    u2_i = (np.dot(M_ic, u_cv) ** 2).sum(1)
+    i_d = u2_i.argsort()
+    m_mv = np.array([(2, 0, 0), (0, 2, 0), (0, 0, 2),
+                     (0, 1, 1), (1, 0, 1), (1, 1, 0)])
+    for D in range(3, 7):
+        h_dv = np.dot(M_ic[i_d[:D]], h_cv)
+        A_md = (h_dv ** m_mv[:, None, :]).prod(2)
+        a_d, residual, rank, s = np.linalg.lstsq(A_md, [1, 1, 1, 0, 0, 0],
+                                                 rcond=-1)
+        if residual.sum() < 1e-14:
+            if rank != D:
+                raise ValueError('You have a weird unit cell!  Try to use the '
+                                 'maximally reduced Niggli cell.')
+            break
+    a_d = a_d * scale
+    offsets = [(0, 0, 0)]
+    coefs = [laplace[n][0] * a_d.sum()]
+    for d in range(D):
+        M_c = M_ic[i_d[d]]
+        offsets.extend(np.arange(1, n + 1)[:, None] * M_c)
+        coefs.extend(a_d[d] * np.array(laplace[n][1:]))
+        offsets.extend(np.arange(-1, -n - 1, -1)[:, None] * M_c)
+        coefs.extend(a_d[d] * np.array(laplace[n][1:]))
+    return np.array(coefs), np.array(offsets), D
+
+
+class GUCLaplace(FDOperator):
+    def __init__(self, gd, scale=1.0, n=1, dtype=float, xp=None):
+        """Laplacian for general non orthorhombic grid; scale=-0.5 gives the
+        kinetic energy operator, n the range (error O(h^2n))."""
+        coefs, offsets, D = guc_laplace_stencil(gd.h_cv, scale, n)
+        FDOperator.__init__(self, coefs, offsets, gd, dtype, xp=xp)
+        self.description = (
+            '%d*%d+1=%d point O(h^%d) finite-difference Laplacian' %
+            ((self.npoints - 1) // n, n, self.npoints, 2 * n))

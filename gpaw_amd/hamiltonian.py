@@ -1,0 +1,66 @@
+"""Finite-difference Hamiltonian applied to wave functions on the GPU.
+
+H psi = T psi + v psi + sum_a |p_a> dH_a <p_a|psi>, as in the reference's
+Hamiltonian.apply (gpaw/hamiltonian.py:445-479): kinetic stencil
+(wfs.kin.apply), local potential (apply_local_potential :437-443, fused into
+the stencil epilogue here), projections P = <p|psi> (pt.integrate), the
+non-local matrices (apply_non_local_hamilton :23-44) and pt.add.
+"""
+import numpy as np
+import torch
+
+from . import _lib
+from ._arrays import Staged, real_dtype, stream_ptr, torch_dtype
+
+
+class FDHamiltonian:
+    def __init__(self, gd, kin, vt_G, pt=None, dH_aii=None, dtype=float):
+        self.gd = gd
+        self.kin = kin
+        self.dtype = dtype
+        self.vt_G = torch.as_tensor(np.asarray(vt_G) if not isinstance(
+            vt_G, torch.Tensor) else vt_G, device='cuda').to(
+                real_dtype(dtype)).contiguous()
+        self.pt = pt
+        self._lib = _lib.lib()
+        if pt is not None:
+            assert dH_aii is not None
+            blocks = []
+            for a in range(len(pt.Mmax_a)):
+                dH = np.asarray(dH_aii[a], float)
+                assert dH.shape == (pt.Mmax_a[a],) * 2
+                blocks.append(dH.ravel())
+            self.dH_flat = torch.as_tensor(np.concatenate(blocks),
+                                           device='cuda')
+            self._dH_host = dH_aii
+
+    def dH(self, P_xM, out=None, eps_x=None, dS_flat=None):
+        """out[x, M_a + i] = sum_j P[x, M_a + j] dH_a[j, i]
+        (- eps_x[x] sum_j P dS_a[j, i])."""
+        if out is None:
+            out = torch.empty_like(P_xM)
+        nb = int(np.prod(P_xM.shape[:-1], dtype=np.int64))
+        eps = None
+        if eps_x is not None:
+            eps = torch.as_tensor(eps_x, dtype=torch.float64,
+                                  device='cuda').contiguous()
+        _lib.check(self._lib.gh_lfc_apply_dH(
+            self.pt.handle, P_xM.data_ptr(), self.dH_flat.data_ptr(),
+            eps.data_ptr() if eps is not None else None,
+            dS_flat.data_ptr() if dS_flat is not None else None,
+            out.data_ptr(), nb, stream_ptr()))
+        return out
+
+    def apply_local(self, psit_xG, out_xG, phase_cd=None):
+        """out = T psi + v psi (one fused stencil pass)."""
+        self.kin.apply(psit_xG, out_xG, phase_cd, vt_g=self.vt_G)
+
+    def apply(self, psit_xG, out_xG, phase_cd=None, P_xM=None):
+        """out = H psi; returns the projections P = <p|psi>."""
+        self.apply_local(psit_xG, out_xG, phase_cd)
+        if self.pt is None:
+            return None
+        if P_xM is None:
+            P_xM = self.pt.integrate_M(psit_xG)
+        self.pt.add_M(out_xG, self.dH(P_xM))
+        return P_xM

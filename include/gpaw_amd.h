@@ -1,0 +1,202 @@
+/* gpaw_amd.h -- C ABI of the MI355X (gfx950) finite-difference H-psi kernels.
+ *
+ * Plain pointers and sizes only.  Array arguments named *_dev are device
+ * pointers (HBM); everything else is host memory.  Grids are C-contiguous
+ * (nbands, n0, n1, n2) with n2 fastest; complex data is interleaved
+ * (re, im) like numpy complex128/complex64.  Bloch phases phase_cd are host
+ * arrays of 3x2 complex numbers (12 doubles: [c][d][re, im]) with
+ * phase_cd[c][0] applied to data that wraps from the low face of axis c into
+ * the high ghost and phase_cd[c][1] to data wrapping from the high face into
+ * the low ghost (reference c/bc.c:242-257, gpaw/kpt_descriptor.py:283-284);
+ * NULL means all phases are 1.
+ *
+ * Every function returns GH_OK (0) or a negative error code; gh_last_error()
+ * then returns a message.  Work is enqueued on the given HIP stream (NULL =
+ * default stream) and is asynchronous unless stated otherwise.  Handles are
+ * immutable after creation and may be used from several streams at once;
+ * scratch memory is owned by the caller or allocated per handle as stated.
+ */
+#ifndef GPAW_AMD_H
+#define GPAW_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GH_OK 0
+#define GH_ERR_ARG (-1)
+#define GH_ERR_SHAPE (-2)
+#define GH_ERR_HIP (-3)
+#define GH_ERR_UNSUPPORTED (-4)
+#define GH_ERR_ALLOC (-5)
+
+/* element types */
+#define GH_F64 0  /* float64            */
+#define GH_C128 1 /* complex128         */
+#define GH_F32 2  /* float32            */
+#define GH_C64 3  /* complex64          */
+
+typedef struct gh_stencil gh_stencil;
+typedef struct gh_transformer gh_transformer;
+typedef struct gh_lfc gh_lfc;
+
+/* Last error message of the calling thread ("" if none). */
+const char* gh_last_error(void);
+
+/* Library version string and the gfx target it was built for. */
+const char* gh_version(void);
+
+/* ----------------------------------------------------------- stencils --
+ * Replaces cgpaw.Operator (reference c/operators.c:384-425, created by
+ * gpaw/fd_operators.py:83-86) and its apply (c/operators.c:250-321,
+ * bc.c:146-308 halo fill + bmgs/fd.c:9-33).
+ *
+ * coefs[ncoefs], offsets_pc[ncoefs*3]: stencil points as in FDOperator
+ * (fd_operators.py:36-91), offsets in grid units per axis.  n_c: grid shape;
+ * pbc_c: 1 = periodic (Bloch) halo, 0 = zero halo.  dtype: GH_*.
+ * The largest |offset| along an axis must not exceed n_c on that axis
+ * (fd_operators.py:53).
+ */
+int gh_stencil_create(gh_stencil** out, int ncoefs, const double* coefs,
+                      const int64_t* offsets_pc, const int64_t* n_c,
+                      const int32_t* pbc_c, int dtype);
+int gh_stencil_destroy(gh_stencil* h);
+
+/* Coefficient at offset (0,0,0) -- Operator.get_diagonal_element
+ * (operators.c:324-337). */
+double gh_stencil_diagonal(const gh_stencil* h);
+
+/* out = L in  (+ vt .* in if vt_dev) (+ beta * src if src_dev).
+ * in_dev/out_dev: (nbands, n0, n1, n2) of the stencil's dtype; in != out.
+ * vt_dev: local potential (n0, n1, n2), real, same precision as the data
+ * (double for F64/C128, float for F32/C64) -- fused
+ * Hamiltonian.apply_local_potential (hamiltonian.py:437-443).
+ * src_dev: optional (nbands, n0, n1, n2) array of the data type. */
+int gh_stencil_apply(const gh_stencil* h, const void* in_dev, void* out_dev,
+                     int64_t nbands, const double* phase_cd,
+                     const void* vt_dev, const void* src_dev, double beta,
+                     void* stream);
+
+/* Tile choice of the stencil kernel (rows of the (y, z) tile per block and
+ * planes per block along x); 0 = automatic.  Used by the autotuner. */
+int gh_stencil_set_tile(gh_stencil* h, int rows, int planes);
+
+/* -------------------------------------------------------- transformers --
+ * Replaces cgpaw.Transformer (reference c/transformers.c:202-252, created
+ * by gpaw/transformers.py:68-76) and its apply (transformers.c:118-164 ->
+ * bmgs/restrict.c, bmgs/interpolate.c).
+ *
+ * nin_c/nout_c: local grid shapes; nn: 1..4 (K = 2 nn); pad_cd[6]: ghost
+ * widths (low, high) per axis of the padded input; skip_cd[6]: interpolation
+ * skips; pbc_c: periodic axes; interpolate: 0 restrict / 1 interpolate.
+ * These are exactly the values transformers.py:27-47 computes.
+ */
+int gh_transformer_create(gh_transformer** out, const int64_t* nin_c,
+                          const int64_t* nout_c, int nn,
+                          const int64_t* pad_cd, const int64_t* skip_cd,
+                          const int32_t* pbc_c, int interpolate, int dtype);
+int gh_transformer_destroy(gh_transformer* h);
+int gh_transformer_apply(const gh_transformer* h, const void* in_dev,
+                         void* out_dev, int64_t nbands,
+                         const double* phase_cd, void* stream);
+
+/* ----------------------------------------------------------------- LFC --
+ * Replaces cgpaw.LFC (reference c/lfc.c:146-257, built by lfc.py:273-369)
+ * with its integrate (lfc.c:432-505) and add (lfc.c:818-885).
+ *
+ * The caller describes each atom's functions on its grid points as
+ * "entries": for atom a, entries [entry_ptr[a], entry_ptr[a+1]) give the
+ * flattened grid index G_e (into n0*n1*n2), the periodic-image index w_e
+ * (into phase_w, -1 = no phase) and nm_a = M_ptr[a+1]-M_ptr[a] function
+ * values A[e][m] (A_e stored row-major with stride nm_a, atoms concatenated
+ * in order).  M_ptr[natoms+1] is the global function numbering
+ * (lfc.py:277-331: atom-major, spline-major, m-minor).  phase_w[nw] complex
+ * (2 doubles each) is exp(2 pi i k . sdisp_w) (lfc.py:339-340), or NULL for
+ * real data.  dv is the grid volume element.  All inputs are host arrays;
+ * the handle keeps device copies.
+ */
+int gh_lfc_create(gh_lfc** out, int64_t ngpts, int natoms,
+                  const int64_t* entry_ptr, const int32_t* M_ptr,
+                  const int32_t* G_e, const int32_t* w_e, const double* A,
+                  int nw, const double* phase_w, double dv, int dtype);
+int gh_lfc_destroy(gh_lfc* h);
+int gh_lfc_nfunctions(const gh_lfc* h);
+
+/* Per-geometry setup on the host (no device work):
+ * gh_spline_coefs -- replaces cgpaw.Spline construction (c/spline.c:151-165
+ * -> bmgs/spline.c:11-47): f[nbins+1] tabulated on [0, rcut] with
+ * dr = rcut / nbins -> out[4 * (nbins + 1)] cubic coefficients per bin.
+ * gh_spline_to_grid -- replaces cgpaw.spline_to_grid (lfc.c:887-977): for
+ * the box [beg_c, end_c) (grid indices), the points with |r - pos_v| < rcut
+ * in C order get A_out[p*(2l+1) + m] = spline(r) * r^l Y_lm (l <= 2, m order
+ * of bmgs/spherical_harmonics.h) and G_out[p] = flattened index relative to
+ * gdcorner_c in a grid of shape n_c.  h_cv is the 3x3 grid-spacing matrix.
+ * A_out/G_out need room for every box point.  Returns the point count. */
+int gh_spline_coefs(int nbins, double dr, const double* f, double* out);
+int64_t gh_spline_to_grid(int l, int nbins, double dr, const double* coefs,
+                          const int64_t* beg_c, const int64_t* end_c,
+                          const double* pos_v, const double* h_cv,
+                          const int64_t* n_c, const int64_t* gdcorner_c,
+                          double* A_out, int32_t* G_out);
+
+/* c[x, M] = dv * sum_G a[x, G] A_M(G) [* phase]   (written, not added)
+ * a_dev: (nbands, ngpts); c_dev: (nbands, Mmax), same dtype. */
+int gh_lfc_integrate(const gh_lfc* h, const void* a_dev, void* c_dev,
+                     int64_t nbands, void* stream);
+
+/* a[x, G] += sum_M c[x, M] A_M(G) [* conj(phase)]. */
+int gh_lfc_add(const gh_lfc* h, const void* c_dev, void* a_dev,
+               int64_t nbands, void* stream);
+
+/* --------------------------------------------- non-local Hamiltonian --
+ * out[x, M_a + i] = sum_j P[x, M_a + j] dH_a[j][i] for every atom
+ * (hamiltonian.py:23-44, apply_non_local_hamilton).  dH_dev holds the
+ * atoms' nm_a x nm_a real matrices concatenated (row-major); P and out have
+ * the LFC's dtype.  If eps_dev (real, nbands) and dS_dev are given, also
+ * subtracts eps[x] * sum_j P[x, j] dS_a[j][i] (eigensolver.py:161-165). */
+int gh_lfc_apply_dH(const gh_lfc* h, const void* P_dev, const double* dH_dev,
+                    const double* eps_dev, const double* dS_dev,
+                    void* out_dev, int64_t nbands, void* stream);
+
+/* ----------------------------------------------------- preconditioner --
+ * The whole 14-step multigrid V-cycle of reference
+ * c/fd_preconditioner.c:108-353 / gpaw/preconditioner.py:35-93 in one call:
+ * d0 = P(R).  kin0/kin1/kin2: stencils of the three levels; rest0/rest1:
+ * restrictions gd0->gd1->gd2; intp2/intp1: interpolations gd2->gd1->gd0;
+ * step = 0.66666666 / kin0 diagonal.  scratch_dev: caller-owned device
+ * buffer of at least gh_precond_scratch_bytes() bytes.  R_dev and d0_dev
+ * are (nbands, n0, n1, n2) of the stencils' dtype. */
+int64_t gh_precond_scratch_bytes(const gh_stencil* kin0,
+                                 const gh_stencil* kin1,
+                                 const gh_stencil* kin2, int64_t nbands);
+int gh_precond_apply(const gh_stencil* kin0, const gh_stencil* kin1,
+                     const gh_stencil* kin2, const gh_transformer* rest0,
+                     const gh_transformer* rest1,
+                     const gh_transformer* intp2,
+                     const gh_transformer* intp1, double step,
+                     const void* R_dev, void* d0_dev, int64_t nbands,
+                     const double* phase_cd, void* scratch_dev, void* stream);
+
+/* ------------------------------------------------- band-wise BLAS-1 --
+ * multi_axpy (gpaw/utilities/mblas.py:13-32): y[x] += a[x] * x[x] with a
+ * per-band real coefficient array a_dev (nbands doubles).  Data dtype any
+ * of GH_*; n = points per band. */
+int gh_multi_axpy(int dtype, int64_t nbands, int64_t n, const double* a_dev,
+                  const void* x_dev, void* y_dev, void* stream);
+
+/* out[x] = Re sum_G conj(a[x,G]) b[x,G] * scale  (per band, FP64 result
+ * and FP64 accumulation for every input precision, fixed reduction order;
+ * the band-wise integrals of rmmdiis.py:124-130,158-164).  b_dev may equal
+ * a_dev.  work_dev: caller-owned device scratch of GH_DOT_CHUNKS * nbands
+ * doubles. */
+#define GH_DOT_CHUNKS 64
+int gh_multi_dotc(int dtype, int64_t nbands, int64_t n, const void* a_dev,
+                  const void* b_dev, double scale, double* out_dev,
+                  double* work_dev, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GPAW_AMD_H */

@@ -1,0 +1,301 @@
+"""GPU parity of the HIP path against the oracle (run on an MI355X).
+
+Every test compares libgpawhip results (through the drop-in classes and the
+C ABI) with oracle/ on the same seeded inputs.  Tolerances: relative 1e-12
+of the result's max-abs for FP64 / complex128 (the reference's own GPU-vs-CPU
+tolerance for the Laplacian, test_fd_laplace.py:54), 1e-13 for transformers
+and LFC, 2e-5 for FP32.
+"""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip('torch')
+
+pytestmark = pytest.mark.gpu
+
+if not torch.cuda.is_available():
+    pytest.skip('no GPU', allow_module_level=True)
+
+from gpaw_amd.fd_operators import FDOperator, Laplace  # noqa: E402
+from gpaw_amd.grid_descriptor import GridDescriptor  # noqa: E402
+from gpaw_amd.transformers import Transformer  # noqa: E402
+from gpaw_amd.preconditioner import Preconditioner  # noqa: E402
+from gpaw_amd.lfc import LFC  # noqa: E402
+from gpaw_amd.spline import Spline  # noqa: E402
+from gpaw_amd.hamiltonian import FDHamiltonian  # noqa: E402
+
+from oracle import grid as og  # noqa: E402
+from oracle import stencil as ost  # noqa: E402
+from oracle import transformers as otr  # noqa: E402
+from oracle import precond as opc  # noqa: E402
+from oracle import lfc as olfc  # noqa: E402
+from oracle import spline as osp  # noqa: E402
+
+
+def dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def host(t):
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+def relerr(a, b):
+    return abs(a - b).max() / max(abs(b).max(), 1e-300)
+
+
+def rand(rng, shape, dtype):
+    a = rng.standard_normal(shape)
+    if np.dtype(dtype).kind == 'c':
+        a = a + 1j * rng.standard_normal(shape)
+    return a.astype(dtype)
+
+
+STENCIL_CASES = [
+    # N_c, h, pbc, n, dtype, k_c
+    ((48, 48, 48), 0.2 / 0.52917721067, True, 3, np.float64, None),
+    ((20, 24, 28), 0.35, True, 3, np.complex128, (0.25, 0, 0)),
+    ((20, 24, 28), 0.35, True, 3, np.complex128, (0.1, -0.3, 0.45)),
+    ((16, 17, 18), 0.4, (True, False, True), 3, np.float64, None),
+    ((16, 18, 20), 0.4, False, 2, np.float64, None),
+    ((12, 14, 16), 0.4, True, 1, np.float64, None),
+    ((14, 12, 30), 0.3, True, 6, np.float64, None),
+    ((14, 12, 30), 0.3, True, 5, np.complex128, (0.2, 0.1, 0.3)),
+    ((10, 12, 200), 0.3, True, 4, np.float64, None),
+    ((20, 24, 28), 0.35, True, 3, np.float32, None),
+    ((20, 24, 28), 0.35, True, 3, np.complex64, (0.25, 0, 0)),
+]
+
+
+@pytest.mark.parametrize('N_c,h,pbc,n,dtype,k_c', STENCIL_CASES)
+def test_laplace_vs_oracle(N_c, h, pbc, n, dtype, k_c):
+    rng = np.random.default_rng(1)
+    gd = GridDescriptor(N_c, np.array(N_c) * h, pbc)
+    ogd = og.Grid(N_c, np.array(N_c) * h, pbc)
+    nb = 4
+    a = rand(rng, (nb,) + tuple(gd.n_c), dtype)
+    ph = None if k_c is None else ogd.phase_cd(k_c)
+    op = Laplace(gd, -0.5, n, dtype)
+    out = torch.empty_like(dev(a))
+    op.apply(dev(a), out, ph)
+    c, o = ost.laplace_stencil(ogd.h_cv, -0.5, n)
+    ref = ost.fd_apply(c, o, a.astype(np.complex128 if np.dtype(dtype).kind
+                                      == 'c' else np.float64), ogd.pbc_c, ph)
+    tol = 2e-5 if dtype in (np.float32, np.complex64) else 1e-12
+    assert relerr(host(out), ref) < tol
+
+
+def test_laplace_fused_potential_and_source():
+    rng = np.random.default_rng(2)
+    N_c = (24, 20, 28)
+    gd = GridDescriptor(N_c, np.array(N_c) * 0.3)
+    ogd = og.Grid(N_c, np.array(N_c) * 0.3)
+    a = rand(rng, (3,) + N_c, np.complex128)
+    src = rand(rng, (3,) + N_c, np.complex128)
+    vt = rng.standard_normal(N_c)
+    ph = ogd.phase_cd((0.25, 0.0, 0.1))
+    op = Laplace(gd, -0.5, 3, complex)
+    out = torch.empty_like(dev(a))
+    op.apply(dev(a), out, ph, vt_g=dev(vt), src_xg=dev(src), beta=-0.7)
+    c, o = ost.laplace_stencil(ogd.h_cv, -0.5, 3)
+    ref = ost.fd_apply(c, o, a, ogd.pbc_c, ph) + vt * a - 0.7 * src
+    assert relerr(host(out), ref) < 1e-12
+
+
+@pytest.mark.parametrize('cell', ['fcc', 'hex'])
+def test_generic_stencil_nonorthogonal(cell):
+    rng = np.random.default_rng(3)
+    if cell == 'fcc':
+        cell_cv = 4.0 * np.array([(0, 1, 1), (1, 0, 1), (1, 1, 0)])
+    else:
+        cell_cv = np.array([(6, 0, 0), (3, 3 * 3**0.5, 0), (0, 0, 7)])
+    N_c = (16, 16, 18)
+    gd = GridDescriptor(N_c, cell_cv)
+    ogd = og.Grid(N_c, cell_cv)
+    op = Laplace(gd, -0.5, 2, complex)
+    assert not op.cfd
+    a = rand(rng, (2,) + N_c, np.complex128)
+    ph = ogd.phase_cd((0.25, 0.1, 0.0))
+    out = torch.empty_like(dev(a))
+    op.apply(dev(a), out, ph)
+    c, o = ost.laplace_stencil(ogd.h_cv, -0.5, 2)
+    assert relerr(host(out), ost.fd_apply(c, o, a, ogd.pbc_c, ph)) < 1e-12
+
+
+def test_stencil_numpy_interface():
+    """NumPy arrays in, NumPy arrays out (staged through the GPU)."""
+    rng = np.random.default_rng(4)
+    N_c = (12, 12, 12)
+    gd = GridDescriptor(N_c, np.array(N_c) * 0.4)
+    a = rng.standard_normal((2,) + N_c)
+    out = np.zeros_like(a)
+    Laplace(gd, -0.5, 3).apply(a, out)
+    c, o = ost.laplace_stencil(og.Grid(N_c, np.array(N_c) * 0.4).h_cv,
+                               -0.5, 3)
+    assert relerr(out, ost.fd_apply(c, o, a, [True] * 3)) < 1e-12
+
+
+def test_stencil_rejects_aliasing():
+    gd = GridDescriptor((8, 8, 8), (3, 3, 3))
+    a = torch.zeros((1, 8, 8, 8), dtype=torch.float64, device='cuda')
+    with pytest.raises(ValueError):
+        Laplace(gd, -0.5, 3).apply(a, a)
+
+
+TRANSFORMER_CASES = [(nn, dt, pbc, k) for nn in (1, 2, 3, 4)
+                     for dt in (np.float64, np.complex128)
+                     for pbc in (True, (True, False, True))
+                     for k in ((0.25, -0.1, 0.3),)]
+
+
+@pytest.mark.parametrize('nn,dtype,pbc,k_c', TRANSFORMER_CASES)
+def test_transformers_vs_oracle(nn, dtype, pbc, k_c):
+    rng = np.random.default_rng(5)
+    N_c = np.array([16, 20, 24])
+    gd0 = GridDescriptor(N_c, N_c * 0.3, pbc)
+    gd1 = gd0.coarsen()
+    ogd0 = og.Grid(N_c, N_c * 0.3, pbc)
+    ogd1 = ogd0.coarsen()
+    ph = ogd0.phase_cd(k_c) if np.dtype(dtype).kind == 'c' else None
+    for gin, gout, oin, oout in ((gd0, gd1, ogd0, ogd1),
+                                 (gd1, gd0, ogd1, ogd0)):
+        a = rand(rng, (3,) + tuple(gin.n_c), dtype)
+        T = Transformer(gin, gout, nn, dtype)
+        out = T.apply(dev(a), phases=ph)
+        ref = otr.Transformer(oin, oout, nn).apply(a, ph)
+        assert relerr(host(out), ref) < 1e-13
+
+
+@pytest.mark.parametrize('dtype', [np.float32, np.complex64])
+def test_transformers_single(dtype):
+    rng = np.random.default_rng(6)
+    N_c = np.array([16, 16, 20])
+    gd0 = GridDescriptor(N_c, N_c * 0.3)
+    ogd0 = og.Grid(N_c, N_c * 0.3)
+    ph = ogd0.phase_cd((0.25, 0, 0)) if np.dtype(dtype).kind == 'c' else None
+    a = rand(rng, (2,) + tuple(N_c), dtype)
+    out = Transformer(gd0, gd0.coarsen(), 3, dtype).apply(dev(a), phases=ph)
+    ref = otr.Transformer(ogd0, ogd0.coarsen(), 3).apply(
+        a.astype(np.complex128 if ph is not None else np.float64), ph)
+    assert relerr(host(out), ref) < 2e-6
+
+
+@pytest.mark.parametrize('dtype,k_c', [(np.float64, None),
+                                       (np.complex128, (0.25, 0, 0)),
+                                       (np.complex128, (0.1, 0.2, -0.3))])
+def test_precond_vs_oracle(dtype, k_c):
+    rng = np.random.default_rng(7)
+    N_c = np.array([24, 28, 32])
+    h = 0.3
+    gd0 = GridDescriptor(N_c, N_c * h)
+    ogd0 = og.Grid(N_c, N_c * h)
+    kin0 = Laplace(gd0, -0.5, 3, dtype)
+    P = Preconditioner(gd0, kin0, dtype)
+    R = rand(rng, (5,) + tuple(N_c), dtype)
+
+    class K:
+        phase_cd = None if k_c is None else ogd0.phase_cd(k_c)
+    d = P(dev(R), K)
+    c0, o0 = ost.laplace_stencil(ogd0.h_cv, -0.5, 3)
+    ref = opc.Preconditioner(ogd0, c0, o0)(R, K.phase_cd)
+    assert relerr(host(d), ref) < 1e-12
+
+
+def _lfc_pair(rng, N_c, h, natoms, dtype, k_c):
+    gd = GridDescriptor(N_c, np.array(N_c) * h)
+    ogd = og.Grid(N_c, np.array(N_c) * h)
+    r = np.linspace(0, 2.0, 101)
+    spl, ospl = [], []
+    for a in range(natoms):
+        rc = rng.uniform(0.6, 1.0, 4)
+        fs = [(l, np.exp(-r**2 / rc[j]**2)) for j, l in
+              enumerate((0, 0, 1, 1))]
+        spl.append([Spline.from_data(l, 2.0, f) for l, f in fs])
+        ospl.append([osp.Spline(l, 2.0, f) for l, f in fs])
+    spos = rng.uniform(0, 1, (natoms, 3))
+    spos[0] = (0.01, 0.98, 0.5)  # crosses periodic faces
+    kd = None if k_c is None else np.array([k_c])
+    lfc = LFC(gd, spl, kd, dtype=dtype)
+    lfc.set_positions(spos)
+    olf = olfc.LFC(ogd, ospl, k_c, dtype)
+    olf.set_positions(spos)
+    return gd, lfc, olf
+
+
+@pytest.mark.parametrize('dtype,k_c', [(np.float64, None),
+                                       (np.complex128, (0.25, 0.1, 0.0))])
+def test_lfc_vs_oracle(dtype, k_c):
+    rng = np.random.default_rng(8)
+    N_c = (20, 22, 24)
+    gd, lfc, olf = _lfc_pair(rng, N_c, 0.38, 6, dtype, k_c)
+    assert lfc.Mmax == olf.Mmax == 48
+    a = rand(rng, (5,) + N_c, dtype)
+    c = lfc.integrate_M(dev(a))
+    ref = olf.integrate(a)
+    assert relerr(host(c), ref) < 1e-13
+    coef = rand(rng, ref.shape, dtype)
+    b = dev(a.copy())
+    lfc.add_M(b, dev(coef))
+    refb = olf.add(a.copy(), coef)
+    assert relerr(host(b), refb) < 1e-13
+    # reference dict interface
+    c_axi = lfc.dict(5)
+    lfc.integrate(dev(a), c_axi, q=0 if k_c else -1)
+    for at in range(6):
+        M = olf.M_a[at]
+        assert relerr(host(c_axi[at]), ref[:, M:M + 8]) < 1e-13
+
+
+def test_lfc_deterministic():
+    """test_lf.py:8-25 -- add twice gives identical bits."""
+    rng = np.random.default_rng(9)
+    gd, lfc, olf = _lfc_pair(rng, (24, 24, 24), 0.38, 10, np.float64, None)
+    coef = dev(rng.standard_normal((3, lfc.Mmax)))
+    x1 = torch.zeros((3, 24, 24, 24), dtype=torch.float64, device='cuda')
+    x2 = torch.zeros_like(x1)
+    lfc.add_M(x1, coef)
+    lfc.add_M(x2, coef)
+    assert torch.equal(x1, x2)
+
+
+def test_full_hpsi_vs_oracle():
+    from gpaw_amd import synthetic as syn
+    rng = np.random.default_rng(10)
+    N = 28
+    w = syn.full_hpsi_geometry(N, 6)
+    gd = GridDescriptor((N,) * 3, (w.L,) * 3)
+    ogd = og.Grid((N,) * 3, (w.L,) * 3)
+    spl = [[Spline.from_data(l, rc, f) for l, rc, f in p] for p in w.proj]
+    ospl = [[osp.Spline(l, rc, f) for l, rc, f in p] for p in w.proj]
+    pt = LFC(gd, spl)
+    pt.set_positions(w.spos_ac)
+    opt = olfc.LFC(ogd, ospl)
+    opt.set_positions(w.spos_ac)
+    vt = syn.potential(gd, w.pos_av)
+    kin = Laplace(gd, -0.5, 3)
+    H = FDHamiltonian(gd, kin, vt, pt, w.dH)
+    psi = rng.standard_normal((4, N, N, N))
+    out = torch.empty_like(dev(psi))
+    H.apply(dev(psi), out)
+    c, o = ost.laplace_stencil(ogd.h_cv, -0.5, 3)
+    ref = ost.fd_apply(c, o, psi, ogd.pbc_c) + vt * psi
+    P = opt.integrate(psi)
+    Q = np.concatenate([P[:, 8 * a:8 * a + 8] @ w.dH[a]
+                        for a in range(6)], axis=1)
+    ref = opt.add(ref, Q)
+    assert relerr(host(out), ref) < 1e-12
+
+
+def test_multi_dotc_and_axpy():
+    from gpaw_amd import blas
+    rng = np.random.default_rng(11)
+    a = rand(rng, (7, 1000), np.complex128)
+    b = rand(rng, (7, 1000), np.complex128)
+    d = blas.multi_dotc(dev(a), dev(b), scale=0.5)
+    ref = 0.5 * np.einsum('xg,xg->x', a.conj(), b).real
+    np.testing.assert_allclose(host(d), ref, rtol=1e-13)
+    lam = rng.standard_normal(7)
+    y = dev(b.copy())
+    blas.multi_axpy(dev(lam), dev(a), y)
+    np.testing.assert_allclose(host(y), b + lam[:, None] * a, rtol=1e-14)
