@@ -41,6 +41,9 @@ def parse():
     p.add_argument('--natoms', type=int, default=64)
     p.add_argument('--cpu-bands', type=int, default=32)
     p.add_argument('--no-cpu', action='store_true')
+    p.add_argument('--calibrate-copy', action='store_true',
+                   help='also run 3 plain device copies of one band array '
+                        '(known bytes) to calibrate HBM counters')
     return p.parse_args()
 
 
@@ -129,6 +132,12 @@ def main():
     for _ in range(args.warmup):
         H.apply(psi, out)
     torch.cuda.synchronize()
+    if args.calibrate_copy:
+        # reads and writes nbands * N^3 * 8 bytes each: the reference
+        # kernel for scaling FETCH_SIZE / WRITE_SIZE counters
+        for _ in range(3):
+            out.copy_(psi)
+        torch.cuda.synchronize()
 
     ev = [(torch.cuda.Event(enable_timing=True),
            torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]

@@ -6,16 +6,22 @@
 //   a[x, G] += sum_M c[x, M] A_M(G) [* conj(phase_W)]
 // where the reference sweeps a sorted event list of sphere z-runs per band.
 //
-// Layout here (built once per geometry by the host):
-//   * atom-major entries: for atom a, entries e in [entry_ptr[a],
-//     entry_ptr[a+1]) carry the grid index G_e, the periodic-image index w_e
-//     and the atom's nm_a function values A[e][0..nm_a) (contiguous);
-//   * a point-major CSR over the union of sphere points: for each distinct G
-//     (sorted), the list of entries that touch it.
-// integrate runs one block per (atom, band tile) and reduces over the atom's
-// entries in LDS -- a gathered skinny GEMM, deterministic; add runs one
-// thread per distinct point and band tile and sums its entries in a fixed
-// order -- no atomics, deterministic, every point written once.
+// Layout here (built once per geometry by the host): atom-major entries --
+// for atom a, entries e in [entry_ptr[a], entry_ptr[a+1]) carry the grid
+// index G_e, the periodic-image index w_e and the atom's nm_a function
+// values A[e][0..nm_a) (contiguous rows).
+//
+// integrate (a gathered skinny GEMM P = psi_a A_a): one block per (atom,
+// band tile); inside a wavefront lane l handles function m = l % nm of entry
+// group l / nm, so each lane keeps one accumulator per band and a wave reads
+// whole A rows with one coalesced load; partial sums are reduced through LDS
+// in a fixed order (deterministic).
+// add (the transposed product scattered back): atoms whose spheres share a
+// grid point get different colours (greedy colouring on the host); one
+// launch per colour, one block per (atom, band tile) with the atom's
+// coefficients staged in LDS.  Atoms of one colour touch disjoint points, so
+// every read-modify-write is race free without atomics, and the order of
+// the contributions to a point is fixed (deterministic).
 #include <algorithm>
 #include <cmath>
 #include <numeric>
@@ -25,12 +31,18 @@
 
 namespace gh {
 
-constexpr int kLfcBT = 4;      // bands per integrate block
-constexpr int kLfcMC = 8;      // functions per integrate pass
-constexpr int kLfcThreads = 256;
+constexpr int kLfcMaxNm = 64;   // functions per atom supported
+constexpr int kIntBT = 4;       // bands per integrate wavefront
+constexpr int kIntMC = 8;       // functions per integrate pass
+constexpr int kAddBT = 8;       // bands per add wavefront
+constexpr int kAddChunk = 512;  // entries per add wavefront
 
+// One wavefront per (atom, band tile, function chunk): lane = entry, so the
+// gathers of a[x, G_e] follow the sphere's z-runs (coalesced) and each lane
+// reads its A row contiguously; 32 partial sums per lane are reduced across
+// the wavefront once at the end (fixed butterfly order).
 template <typename V, typename S>
-__global__ void __launch_bounds__(kLfcThreads)
+__global__ void __launch_bounds__(64)
 lfc_integrate_kernel(const V* __restrict__ a, V* __restrict__ c,
                      const int64_t* __restrict__ entry_ptr,
                      const int32_t* __restrict__ M_ptr,
@@ -39,114 +51,107 @@ lfc_integrate_kernel(const V* __restrict__ a, V* __restrict__ c,
                      const int32_t* __restrict__ w_e,
                      const S* __restrict__ A, const S* __restrict__ phase_w,
                      int64_t ngpts, int Mmax, int nbands, S dv) {
-    __shared__ V red[kLfcThreads / 64][kLfcBT * kLfcMC];
     const int atom = blockIdx.x;
-    const int x0 = blockIdx.y * kLfcBT;
-    const int e0 = (int)entry_ptr[atom];
-    const int e1 = (int)entry_ptr[atom + 1];
+    const int x0 = blockIdx.y * kIntBT;
     const int M0 = M_ptr[atom];
     const int nm = M_ptr[atom + 1] - M0;
-    const S* Aa = A + A_off[atom];
-    const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
+    const int m0 = blockIdx.z * kIntMC;
+    if (m0 >= nm) return;
+    const int mc = min(kIntMC, nm - m0);
+    const int64_t e0 = entry_ptr[atom];
+    const int64_t e1 = entry_ptr[atom + 1];
+    const S* __restrict__ Aa = A + A_off[atom];
+    const int lane = threadIdx.x;
+    const int nb = min(kIntBT, nbands - x0);
 
-    for (int m0 = 0; m0 < nm; m0 += kLfcMC) {
-        const int mc = min(kLfcMC, nm - m0);
-        V acc[kLfcBT][kLfcMC];
+    V acc[kIntBT][kIntMC];
 #pragma unroll
-        for (int b = 0; b < kLfcBT; b++)
+    for (int b = 0; b < kIntBT; b++)
 #pragma unroll
-            for (int m = 0; m < kLfcMC; m++) acc[b][m] = vzero<V>();
-        for (int e = e0 + threadIdx.x; e < e1; e += kLfcThreads) {
-            const int64_t G = G_e[e];
-            V av[kLfcBT];
-            S pre = 1, pim = 0;
-            if (is_complex<V>::value && phase_w) {
-                int w = w_e[e];
-                if (w >= 0) { pre = phase_w[2 * w]; pim = phase_w[2 * w + 1]; }
-            }
+        for (int m = 0; m < kIntMC; m++) acc[b][m] = vzero<V>();
+    for (int64_t e = e0 + lane; e < e1; e += 64) {
+        const int64_t G = G_e[e];
+        S pre = 1, pim = 0;
+        if (is_complex<V>::value && phase_w) {
+            int w = w_e[e];
+            if (w >= 0) { pre = phase_w[2 * w]; pim = phase_w[2 * w + 1]; }
+        }
+        V av[kIntBT];
 #pragma unroll
-            for (int b = 0; b < kLfcBT; b++) {
-                int x = x0 + b;
-                av[b] = x < nbands ? mul_phase(a[(size_t)x * ngpts + G], pre, pim)
-                                   : vzero<V>();
-            }
-            const S* Ae = Aa + (size_t)(e - e0) * nm + m0;
+        for (int b = 0; b < kIntBT; b++)
+            av[b] = b < nb ? mul_phase(a[(size_t)(x0 + b) * ngpts + G], pre, pim)
+                           : vzero<V>();
+        const S* __restrict__ Ae = Aa + (e - e0) * nm + m0;
 #pragma unroll
-            for (int m = 0; m < kLfcMC; m++) {
-                if (m < mc) {
-                    S Am = Ae[m];
+        for (int m = 0; m < kIntMC; m++) {
+            if (m < mc) {
+                const S Am = Ae[m];
 #pragma unroll
-                    for (int b = 0; b < kLfcBT; b++) acc[b][m] = fma_rs(Am, av[b], acc[b][m]);
-                }
+                for (int b = 0; b < kIntBT; b++) acc[b][m] = fma_rs(Am, av[b], acc[b][m]);
             }
         }
-        // wave reduction (64 lanes) then across waves through LDS
-#pragma unroll
-        for (int b = 0; b < kLfcBT; b++)
-#pragma unroll
-            for (int m = 0; m < kLfcMC; m++) {
-                V v = acc[b][m];
-#pragma unroll
-                for (int off = 32; off > 0; off >>= 1) {
-                    if constexpr (is_complex<V>::value) {
-                        v.re += __shfl_xor(v.re, off);
-                        v.im += __shfl_xor(v.im, off);
-                    } else {
-                        v += __shfl_xor(v, off);
-                    }
-                }
-                if (lane == 0) red[wave][b * kLfcMC + m] = v;
-            }
-        __syncthreads();
-        if (threadIdx.x < kLfcBT * kLfcMC) {
-            int b = threadIdx.x / kLfcMC;
-            int m = threadIdx.x % kLfcMC;
-            V s = red[0][threadIdx.x];
-            for (int w = 1; w < kLfcThreads / 64; w++) s = vadd(s, red[w][threadIdx.x]);
-            int x = x0 + b;
-            if (x < nbands && m < mc) c[(size_t)x * Mmax + M0 + m0 + m] = vscale(s, dv);
-        }
-        __syncthreads();
     }
+#pragma unroll
+    for (int b = 0; b < kIntBT; b++)
+#pragma unroll
+        for (int m = 0; m < kIntMC; m++) {
+            V v = acc[b][m];
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) {
+                if constexpr (is_complex<V>::value) {
+                    v.re += __shfl_xor(v.re, off);
+                    v.im += __shfl_xor(v.im, off);
+                } else {
+                    v += __shfl_xor(v, off);
+                }
+            }
+            if (lane == b * kIntMC + m && b < nb && m < mc)
+                c[(size_t)(x0 + b) * Mmax + M0 + m0 + m] = vscale(v, dv);
+        }
 }
 
-constexpr int kAddBT = 4;
-
+// One wavefront per (atom of the colour, entry chunk, band tile): lane =
+// entry; the atom's coefficients for the band tile are staged in LDS and
+// read as broadcasts.
 template <typename V, typename S>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(64)
 lfc_add_kernel(const V* __restrict__ c, V* __restrict__ a,
-               const int32_t* __restrict__ uG, const int32_t* __restrict__ uptr,
-               const int32_t* __restrict__ uref,
-               const int32_t* __restrict__ atom_of_e,
+               const int32_t* __restrict__ atoms, int nchunks,
+               const int64_t* __restrict__ entry_ptr,
                const int32_t* __restrict__ M_ptr,
                const int64_t* __restrict__ A_off,
-               const int64_t* __restrict__ entry_ptr,
+               const int32_t* __restrict__ G_e,
                const int32_t* __restrict__ w_e, const S* __restrict__ A,
-               const S* __restrict__ phase_w, int nu, int64_t ngpts, int Mmax,
+               const S* __restrict__ phase_w, int64_t ngpts, int Mmax,
                int nbands) {
-    const int u = blockIdx.x * blockDim.x + threadIdx.x;
-    if (u >= nu) return;
+    __shared__ V cl[kAddBT][kLfcMaxNm];
+    const int atom = atoms[blockIdx.x / nchunks];
+    const int chunk = blockIdx.x % nchunks;
     const int x0 = blockIdx.y * kAddBT;
-    V acc[kAddBT];
+    const int64_t e0 = entry_ptr[atom];
+    const int64_t eb = e0 + (int64_t)chunk * kAddChunk;
+    const int64_t e1 = min(entry_ptr[atom + 1], eb + kAddChunk);
+    if (eb >= e1) return;
+    const int M0 = M_ptr[atom];
+    const int nm = M_ptr[atom + 1] - M0;
+    const int nb = min(kAddBT, nbands - x0);
+    for (int t = threadIdx.x; t < kAddBT * nm; t += 64) {
+        const int b = t / nm;
+        const int m = t - b * nm;
+        cl[b][m] = b < nb ? c[(size_t)(x0 + b) * Mmax + M0 + m] : vzero<V>();
+    }
+    __syncthreads();
+    const S* __restrict__ Aa = A + A_off[atom];
+    for (int64_t e = eb + threadIdx.x; e < e1; e += 64) {
+        const int64_t G = G_e[e];
+        const S* __restrict__ Ae = Aa + (e - e0) * nm;
+        V s[kAddBT];
 #pragma unroll
-    for (int b = 0; b < kAddBT; b++) acc[b] = vzero<V>();
-    for (int r = uptr[u]; r < uptr[u + 1]; r++) {
-        const int e = uref[r];
-        const int at = atom_of_e[e];
-        const int M0 = M_ptr[at];
-        const int nm = M_ptr[at + 1] - M0;
-        const S* Ae = A + A_off[at] + (size_t)(e - entry_ptr[at]) * nm;
-        V part[kAddBT];
-#pragma unroll
-        for (int b = 0; b < kAddBT; b++) part[b] = vzero<V>();
+        for (int b = 0; b < kAddBT; b++) s[b] = vzero<V>();
         for (int m = 0; m < nm; m++) {
-            S Am = Ae[m];
+            const S Am = Ae[m];
 #pragma unroll
-            for (int b = 0; b < kAddBT; b++) {
-                int x = x0 + b;
-                if (x < nbands) part[b] = fma_rs(Am, c[(size_t)x * Mmax + M0 + m], part[b]);
-            }
+            for (int b = 0; b < kAddBT; b++) s[b] = fma_rs(Am, cl[b][m], s[b]);
         }
         S pre = 1, pim = 0;
         if (is_complex<V>::value && phase_w) {
@@ -154,15 +159,11 @@ lfc_add_kernel(const V* __restrict__ c, V* __restrict__ a,
             if (w >= 0) { pre = phase_w[2 * w]; pim = -phase_w[2 * w + 1]; }
         }
 #pragma unroll
-        for (int b = 0; b < kAddBT; b++) acc[b] = vadd(acc[b], mul_phase(part[b], pre, pim));
-    }
-    const int64_t G = uG[u];
-#pragma unroll
-    for (int b = 0; b < kAddBT; b++) {
-        int x = x0 + b;
-        if (x < nbands) {
-            V* p = a + (size_t)x * ngpts + G;
-            *p = vadd(*p, acc[b]);
+        for (int b = 0; b < kAddBT; b++) {
+            if (b < nb) {
+                V* p = a + (size_t)(x0 + b) * ngpts + G;
+                *p = vadd(*p, mul_phase(s[b], pre, pim));
+            }
         }
     }
 }
@@ -216,12 +217,12 @@ struct gh_lfc {
     int64_t* dH_off = nullptr;
     int32_t* G_e = nullptr;
     int32_t* w_e = nullptr;
-    int32_t* atom_of_e = nullptr;
     void* A = nullptr;         // double or float
     void* phase_w = nullptr;   // (re, im) pairs, double or float
-    int32_t* uG = nullptr;
-    int32_t* uptr = nullptr;
-    int32_t* uref = nullptr;
+    int max_nm = 0;                     // most functions on one atom
+    int64_t max_entries = 0;            // most entries on one atom
+    int32_t* color_atoms = nullptr;     // atoms grouped by colour
+    std::vector<int> color_ptr;         // host: colour c -> [ptr[c], ptr[c+1])
 };
 
 using namespace gh;
@@ -240,7 +241,7 @@ int upload(T** dst, const T* src, size_t n) {
 void free_lfc(gh_lfc* h) {
     if (!h) return;
     void* ptrs[] = {h->entry_ptr, h->M_ptr, h->A_off, h->dH_off, h->G_e, h->w_e,
-                    h->atom_of_e, h->A, h->phase_w, h->uG, h->uptr, h->uref};
+                    h->A, h->phase_w, h->color_atoms};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     delete h;
@@ -270,6 +271,8 @@ int gh_lfc_create(gh_lfc** out, int64_t ngpts, int natoms,
         int nm = M_ptr[a + 1] - M_ptr[a];
         GH_REQUIRE(nm >= 0 && entry_ptr[a + 1] >= entry_ptr[a], GH_ERR_ARG,
                    "lfc_create: pointers not monotonic at atom %d", a);
+        GH_REQUIRE(nm <= kLfcMaxNm, GH_ERR_UNSUPPORTED,
+                   "lfc_create: %d functions on atom %d (max %d)", nm, a, kLfcMaxNm);
         A_off[a + 1] = A_off[a] + (entry_ptr[a + 1] - entry_ptr[a]) * nm;
         dH_off[a + 1] = dH_off[a] + (int64_t)nm * nm;
         for (int64_t e = entry_ptr[a]; e < entry_ptr[a + 1]; e++) {
@@ -279,28 +282,53 @@ int gh_lfc_create(gh_lfc** out, int64_t ngpts, int natoms,
             atom_of_e[e] = a;
         }
     }
-    // point-major CSR over distinct G (entries in a fixed, sorted order)
+    // colour atoms so that atoms sharing a grid point differ in colour
     std::vector<int64_t> order(ne);
     std::iota(order.begin(), order.end(), 0);
     std::stable_sort(order.begin(), order.end(),
                      [&](int64_t x, int64_t y) { return G_e[x] < G_e[y]; });
-    std::vector<int32_t> uG, uptr, uref;
-    uref.reserve(ne);
-    for (int64_t i = 0; i < ne; i++) {
-        int64_t e = order[i];
-        if (uG.empty() || uG.back() != G_e[e]) {
-            uG.push_back(G_e[e]);
-            uptr.push_back((int32_t)i);
-        }
-        uref.push_back((int32_t)e);
+    std::vector<std::vector<int>> nbr(natoms);
+    int64_t nunique = 0;
+    for (int64_t i = 0; i < ne;) {
+        int64_t j = i;
+        while (j < ne && G_e[order[j]] == G_e[order[i]]) j++;
+        nunique++;
+        for (int64_t p = i; p < j; p++)
+            for (int64_t q = p + 1; q < j; q++) {
+                int aa = atom_of_e[order[p]], bb = atom_of_e[order[q]];
+                if (aa != bb) { nbr[aa].push_back(bb); nbr[bb].push_back(aa); }
+            }
+        i = j;
     }
-    uptr.push_back((int32_t)ne);
+    std::vector<int> color(natoms, -1);
+    int ncolors = 0;
+    for (int a = 0; a < natoms; a++) {
+        std::vector<char> used(ncolors + 1, 0);
+        for (int b : nbr[a])
+            if (color[b] >= 0) used[color[b]] = 1;
+        int c = 0;
+        while (used[c]) c++;
+        color[a] = c;
+        ncolors = std::max(ncolors, c + 1);
+    }
+    std::vector<int32_t> color_atoms;
+    std::vector<int> color_ptr(1, 0);
+    for (int c = 0; c < ncolors; c++) {
+        for (int a = 0; a < natoms; a++)
+            if (color[a] == c && M_ptr[a + 1] > M_ptr[a]) color_atoms.push_back(a);
+        color_ptr.push_back((int)color_atoms.size());
+    }
 
     auto* h = new gh_lfc();
     h->dtype = dtype;
     h->natoms = natoms;
     h->Mmax = M_ptr[natoms];
-    h->nu = (int)uG.size();
+    h->nu = (int)nunique;
+    h->color_ptr = color_ptr;
+    for (int a = 0; a < natoms; a++) {
+        h->max_nm = std::max(h->max_nm, M_ptr[a + 1] - M_ptr[a]);
+        h->max_entries = std::max(h->max_entries, entry_ptr[a + 1] - entry_ptr[a]);
+    }
     h->ngpts = ngpts;
     h->nentries = ne;
     h->dv = dv;
@@ -311,10 +339,7 @@ int gh_lfc_create(gh_lfc** out, int64_t ngpts, int natoms,
     UP(&h->A_off, A_off.data(), A_off.size());
     UP(&h->dH_off, dH_off.data(), dH_off.size());
     UP(&h->G_e, G_e, (size_t)ne);
-    UP(&h->atom_of_e, atom_of_e.data(), (size_t)ne);
-    UP(&h->uG, uG.data(), uG.size());
-    UP(&h->uptr, uptr.data(), uptr.size());
-    UP(&h->uref, uref.data(), uref.size());
+    UP(&h->color_atoms, color_atoms.data(), color_atoms.size());
     std::vector<int32_t> w(ne, -1);
     if (w_e) std::copy(w_e, w_e + ne, w.begin());
     UP(&h->w_e, w.data(), w.size());
@@ -359,15 +384,16 @@ int gh_lfc_integrate(const gh_lfc* h, const void* a_dev, void* c_dev,
     // every (band, M) is written by the block of its atom, also for atoms
     // without grid points, so no clearing pass is needed
     hipStream_t st = (hipStream_t)stream;
-    dim3 grid(h->natoms, (unsigned)((nbands + kLfcBT - 1) / kLfcBT));
+    dim3 grid(h->natoms, (unsigned)((nbands + kIntBT - 1) / kIntBT),
+              (unsigned)((h->max_nm + kIntMC - 1) / kIntMC));
     const bool ph = cplxd(h->dtype) && h->phase_w;
 #define ARGS(V, S) (const V*)a_dev, (V*)c_dev, h->entry_ptr, h->M_ptr, h->A_off, h->G_e, h->w_e, \
                    (const S*)h->A, ph ? (const S*)h->phase_w : nullptr, h->ngpts, h->Mmax, (int)nbands, (S)h->dv
     switch (h->dtype) {
-        case GH_F64: hipLaunchKernelGGL((lfc_integrate_kernel<double, double>), grid, dim3(kLfcThreads), 0, st, ARGS(double, double)); break;
-        case GH_C128: hipLaunchKernelGGL((lfc_integrate_kernel<cplx<double>, double>), grid, dim3(kLfcThreads), 0, st, ARGS(cplx<double>, double)); break;
-        case GH_F32: hipLaunchKernelGGL((lfc_integrate_kernel<float, float>), grid, dim3(kLfcThreads), 0, st, ARGS(float, float)); break;
-        case GH_C64: hipLaunchKernelGGL((lfc_integrate_kernel<cplx<float>, float>), grid, dim3(kLfcThreads), 0, st, ARGS(cplx<float>, float)); break;
+        case GH_F64: hipLaunchKernelGGL((lfc_integrate_kernel<double, double>), grid, dim3(64), 0, st, ARGS(double, double)); break;
+        case GH_C128: hipLaunchKernelGGL((lfc_integrate_kernel<cplx<double>, double>), grid, dim3(64), 0, st, ARGS(cplx<double>, double)); break;
+        case GH_F32: hipLaunchKernelGGL((lfc_integrate_kernel<float, float>), grid, dim3(64), 0, st, ARGS(float, float)); break;
+        case GH_C64: hipLaunchKernelGGL((lfc_integrate_kernel<cplx<float>, float>), grid, dim3(64), 0, st, ARGS(cplx<float>, float)); break;
     }
 #undef ARGS
     GH_CHECK_LAUNCH();
@@ -377,21 +403,28 @@ int gh_lfc_integrate(const gh_lfc* h, const void* a_dev, void* c_dev,
 int gh_lfc_add(const gh_lfc* h, const void* c_dev, void* a_dev,
                int64_t nbands, void* stream) {
     GH_REQUIRE(h && a_dev && c_dev, GH_ERR_ARG, "lfc_add: null argument");
-    if (nbands == 0 || h->nu == 0) return GH_OK;
+    if (nbands == 0 || h->nentries == 0) return GH_OK;
     hipStream_t st = (hipStream_t)stream;
-    dim3 grid((h->nu + 255) / 256, (unsigned)((nbands + kAddBT - 1) / kAddBT));
     const bool ph = cplxd(h->dtype) && h->phase_w;
-#define ARGS(V, S) (const V*)c_dev, (V*)a_dev, h->uG, h->uptr, h->uref, h->atom_of_e, h->M_ptr, \
-                   h->A_off, h->entry_ptr, h->w_e, (const S*)h->A, ph ? (const S*)h->phase_w : nullptr, \
-                   h->nu, h->ngpts, h->Mmax, (int)nbands
-    switch (h->dtype) {
-        case GH_F64: hipLaunchKernelGGL((lfc_add_kernel<double, double>), grid, dim3(256), 0, st, ARGS(double, double)); break;
-        case GH_C128: hipLaunchKernelGGL((lfc_add_kernel<cplx<double>, double>), grid, dim3(256), 0, st, ARGS(cplx<double>, double)); break;
-        case GH_F32: hipLaunchKernelGGL((lfc_add_kernel<float, float>), grid, dim3(256), 0, st, ARGS(float, float)); break;
-        case GH_C64: hipLaunchKernelGGL((lfc_add_kernel<cplx<float>, float>), grid, dim3(256), 0, st, ARGS(cplx<float>, float)); break;
-    }
+    const unsigned ntiles = (unsigned)((nbands + kAddBT - 1) / kAddBT);
+    for (size_t col = 0; col + 1 < h->color_ptr.size(); col++) {
+        const int n = h->color_ptr[col + 1] - h->color_ptr[col];
+        if (n == 0) continue;
+        const int nchunks = (int)((h->max_entries + kAddChunk - 1) / kAddChunk);
+        dim3 grid(n * nchunks, ntiles);
+        const int32_t* atoms = h->color_atoms + h->color_ptr[col];
+#define ARGS(V, S) (const V*)c_dev, (V*)a_dev, atoms, nchunks, h->entry_ptr, h->M_ptr, h->A_off, \
+                   h->G_e, h->w_e, (const S*)h->A, ph ? (const S*)h->phase_w : nullptr, \
+                   h->ngpts, h->Mmax, (int)nbands
+        switch (h->dtype) {
+            case GH_F64: hipLaunchKernelGGL((lfc_add_kernel<double, double>), grid, dim3(64), 0, st, ARGS(double, double)); break;
+            case GH_C128: hipLaunchKernelGGL((lfc_add_kernel<cplx<double>, double>), grid, dim3(64), 0, st, ARGS(cplx<double>, double)); break;
+            case GH_F32: hipLaunchKernelGGL((lfc_add_kernel<float, float>), grid, dim3(64), 0, st, ARGS(float, float)); break;
+            case GH_C64: hipLaunchKernelGGL((lfc_add_kernel<cplx<float>, float>), grid, dim3(64), 0, st, ARGS(cplx<float>, float)); break;
+        }
 #undef ARGS
-    GH_CHECK_LAUNCH();
+        GH_CHECK_LAUNCH();
+    }
     return GH_OK;
 }
 

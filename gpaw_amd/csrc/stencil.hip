@@ -20,7 +20,9 @@
 // Other stencils (non-orthogonal cells) use stencil_generic_kernel, one
 // thread per point reading its neighbours through the cache hierarchy.
 #include <cstdarg>
+#include <cstdlib>
 #include <cstring>
+#include <string>
 #include <vector>
 #include <algorithm>
 
@@ -74,8 +76,13 @@ __device__ __forceinline__ V load_x(const V* __restrict__ inb, int x, int yz,
     return mul_phase(v, ph.re[0][d], ph.im[0][d]);
 }
 
-template <typename V, typename S, int R, int PPT, bool HAS_V, bool HAS_SRC>
-__global__ void __launch_bounds__(1024)
+constexpr int kCfdMaxThreads = 512;
+
+// kGhostSlots: ghost values each thread prefetches per plane (1 when the
+// 2R ghost rows fit in one pass of the block, else 4 plus a tail loop).
+template <typename V, typename S, int R, int PPT, int kGhostSlots, bool HAS_V,
+          bool HAS_SRC>
+__global__ void __launch_bounds__(kCfdMaxThreads)
 stencil_cfd_kernel(const V* __restrict__ in, V* __restrict__ out,
                    const S* __restrict__ vt, const V* __restrict__ src,
                    S beta, CfdCoefs<S> cf, CfdGeom g, PhaseArgs<S> ph) {
@@ -83,6 +90,7 @@ stencil_cfd_kernel(const V* __restrict__ in, V* __restrict__ out,
     V* lds = reinterpret_cast<V*>(smem_raw);
 
     const int n2 = g.n2;
+    const int NT = blockDim.x;
     const int tiles_per_band = g.nyt * g.nxc;
     const int b = xcd_swizzle(blockIdx.x, gridDim.x);
     const int band = b / tiles_per_band;
@@ -99,60 +107,107 @@ stencil_cfd_kernel(const V* __restrict__ in, V* __restrict__ out,
     V* __restrict__ outb = out + band * bstride;
     const V* __restrict__ srcb = HAS_SRC ? src + band * bstride : nullptr;
     const int tile_pts = tye * n2;
-    const int rows = g.ty + 2 * R;
-    const int bufsz = rows * n2;
+    const int bufsz = (g.ty + 2 * R) * n2;
+    const int nghost = 2 * R * n2;
 
-    int yl[PPT], zz[PPT];
+    // own points of this thread: tile index p = tid + k * NT
+    int yl[PPT], zz[PPT], yz[PPT];
     bool act[PPT];
 #pragma unroll
     for (int k = 0; k < PPT; k++) {
-        int p = threadIdx.x + k * blockDim.x;
+        int p = threadIdx.x + k * NT;
         act[k] = p < tile_pts;
         int pp = act[k] ? p : 0;
         yl[k] = pp / n2;
         zz[k] = pp - yl[k] * n2;
+        yz[k] = (y0 + yl[k]) * n2 + zz[k];
     }
-
-    // register queue: q[k][j] holds plane (x - R + j) at the thread's point k
-    V q[PPT][2 * R + 1];
+    // ghost rows (R above, R below the tile): LDS slot, in-plane source
+    // offset and kind (0 interior, 1 wrapped low, 2 wrapped high, 3 zero)
+    int gl[kGhostSlots], gsrc[kGhostSlots], gkind[kGhostSlots];
 #pragma unroll
-    for (int j = 0; j < 2 * R; j++) {
-#pragma unroll
-        for (int k = 0; k < PPT; k++) {
-            int yz = (y0 + yl[k]) * n2 + zz[k];
-            q[k][j] = act[k] ? load_x(inb, x0 - R + j, yz, g, ph) : vzero<V>();
-        }
-    }
-
-    for (int x = x0; x < x1; x++) {
-#pragma unroll
-        for (int k = 0; k < PPT; k++) {
-            int yz = (y0 + yl[k]) * n2 + zz[k];
-            q[k][2 * R] = act[k] ? load_x(inb, x + R, yz, g, ph) : vzero<V>();
-        }
-        V* buf = lds + (x & 1) * bufsz;
-#pragma unroll
-        for (int k = 0; k < PPT; k++)
-            if (act[k]) buf[(yl[k] + R) * n2 + zz[k]] = q[k][R];
-        // ghost rows above and below the tile (y wraps with phase or zero)
-        const V* __restrict__ pl = inb + (size_t)x * plane;
-        for (int i = threadIdx.x; i < 2 * R * n2; i += blockDim.x) {
-            int hr = i / n2;
-            int z = i - hr * n2;
+    for (int i = 0; i < kGhostSlots; i++) {
+        int idx = threadIdx.x + i * NT;
+        gl[i] = -1;
+        gsrc[i] = 0;
+        gkind[i] = 3;
+        if (idx < nghost) {
+            int hr = idx / n2;
+            int z = idx - hr * n2;
             int lrow = hr < R ? hr : hr + tye;
             int y = y0 - R + lrow;
-            V v;
-            if (y >= 0 && y < g.n1) {
-                v = pl[y * n2 + z];
-            } else if (g.pbc1) {
-                int d = y < 0 ? 1 : 0;
-                int yy = y < 0 ? y + g.n1 : y - g.n1;
-                v = mul_phase(pl[yy * n2 + z], ph.re[1][d], ph.im[1][d]);
-            } else {
-                v = vzero<V>();
+            gl[i] = lrow * n2 + z;
+            if (y >= 0 && y < g.n1) { gkind[i] = 0; gsrc[i] = y * n2 + z; }
+            else if (g.pbc1) {
+                gkind[i] = y < 0 ? 1 : 2;
+                gsrc[i] = (y < 0 ? y + g.n1 : y - g.n1) * n2 + z;
             }
-            buf[lrow * n2 + z] = v;
         }
+    }
+
+    auto ghost_value = [&](const V* __restrict__ pl, int src_off, int kind) {
+        if (kind == 0) return pl[src_off];
+        if (kind == 3) return vzero<V>();
+        V v = pl[src_off];
+        return kind == 1 ? mul_phase(v, ph.re[1][1], ph.im[1][1])
+                         : mul_phase(v, ph.re[1][0], ph.im[1][0]);
+    };
+
+    // register queue: q[k][j] holds plane (x - R + j) at own point k
+    V q[PPT][2 * R + 1];
+#pragma unroll
+    for (int j = 0; j < 2 * R; j++)
+#pragma unroll
+        for (int k = 0; k < PPT; k++)
+            q[k][j] = act[k] ? load_x(inb, x0 - R + j, yz[k], g, ph) : vzero<V>();
+
+    // Two-deep prefetch ring: slot s holds the own points of plane
+    // (x + R) and the ghost rows of plane x for the iteration that uses it;
+    // the loads for plane x + 2 are issued as soon as slot s is consumed, so
+    // two planes of loads are in flight behind each barrier.
+    V nq[2][PPT];
+    V ng[2][kGhostSlots];
+    auto prefetch = [&](int xn, V* nqs, V* ngs) {
+#pragma unroll
+        for (int k = 0; k < PPT; k++)
+            nqs[k] = act[k] ? load_x(inb, xn + R, yz[k], g, ph) : vzero<V>();
+        const V* pl = inb + (size_t)xn * plane;
+#pragma unroll
+        for (int i = 0; i < kGhostSlots; i++)
+            ngs[i] = gl[i] >= 0 ? ghost_value(pl, gsrc[i], gkind[i]) : vzero<V>();
+    };
+    prefetch(x0, nq[0], ng[0]);
+    if (x0 + 1 < x1) prefetch(x0 + 1, nq[1], ng[1]);
+
+    auto step = [&](int x, V* nqs, V* ngs) {
+        V* buf = lds + (x & 1) * bufsz;
+#pragma unroll
+        for (int k = 0; k < PPT; k++) {
+            q[k][2 * R] = nqs[k];
+            if (act[k]) buf[(yl[k] + R) * n2 + zz[k]] = q[k][R];
+        }
+#pragma unroll
+        for (int i = 0; i < kGhostSlots; i++)
+            if (gl[i] >= 0) buf[gl[i]] = ngs[i];
+        // ghosts beyond the prefetched slots (very small tiles only)
+        if (nghost > kGhostSlots * NT) {
+            const V* pl = inb + (size_t)x * plane;
+            for (int idx = threadIdx.x + kGhostSlots * NT; idx < nghost; idx += NT) {
+                int hr = idx / n2;
+                int z = idx - hr * n2;
+                int lrow = hr < R ? hr : hr + tye;
+                int y = y0 - R + lrow;
+                V v;
+                if (y >= 0 && y < g.n1) v = pl[y * n2 + z];
+                else if (g.pbc1) {
+                    int d = y < 0 ? 1 : 0;
+                    int yy = y < 0 ? y + g.n1 : y - g.n1;
+                    v = mul_phase(pl[yy * n2 + z], ph.re[1][d], ph.im[1][d]);
+                } else v = vzero<V>();
+                buf[lrow * n2 + z] = v;
+            }
+        }
+        if (x + 2 < x1) prefetch(x + 2, nqs, ngs);
         __syncthreads();
 #pragma unroll
         for (int k = 0; k < PPT; k++) {
@@ -186,10 +241,253 @@ stencil_cfd_kernel(const V* __restrict__ in, V* __restrict__ out,
                 acc = fma_rs(cf.cm[2][j], vm, acc);
                 acc = fma_rs(cf.cp[2][j], vp, acc);
             }
-            const size_t gi = (size_t)x * plane + (size_t)(y0 + yl[k]) * n2 + z;
+            const size_t gi = (size_t)x * plane + yz[k];
             if (HAS_V) acc = fma_rs(vt[gi], q[k][R], acc);
             if (HAS_SRC) acc = fma_rs(beta, srcb[gi], acc);
             outb[gi] = acc;
+        }
+#pragma unroll
+        for (int k = 0; k < PPT; k++)
+#pragma unroll
+            for (int j = 0; j < 2 * R; j++) q[k][j] = q[k][j + 1];
+    };
+
+    for (int x = x0; x < x1; x += 2) {
+        step(x, nq[0], ng[0]);
+        if (x + 1 < x1) step(x + 1, nq[1], ng[1]);
+    }
+}
+
+// ------------------------------------------------- LDS-ring stencil --
+// The high-throughput form of stencil_cfd_kernel.  The block owns one band,
+// a tile of ty full (y, z) rows and a run of x-planes, as above, but plane
+// tiles (ty + 2R rows, ghosts included) are streamed HBM -> LDS by
+// global_load_lds DMA into a ring of P = R + D + 1 slots, D planes ahead of
+// use, by D loader wavefronts: loader w owns the planes xp with
+// (xp - x0) % D == w, so it has exactly one plane in flight and waits for it
+// with vmcnt(0) just before the barrier of the iteration that first reads
+// it.  Compute wavefronts never write LDS: they refill the x register queue
+// from the ring, read y/z neighbours from the current plane's slot, apply
+// Bloch phases / zero ghosts at read time, and store H psi.
+constexpr int kRingMaxThreads = 1024;
+constexpr int kDmaBytes = 16;              // bytes per lane per DMA op
+
+__device__ __forceinline__ void ring_barrier() {
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_barrier();
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
+__device__ __forceinline__ void wait_vmcnt0() {
+    // s_waitcnt vmcnt(0) expcnt(7) lgkmcnt(15): only vector memory
+    __builtin_amdgcn_s_waitcnt(0x0F70);
+}
+
+struct RingGeom {
+    CfdGeom g;
+    int nload;            // loader wavefronts (= prefetch depth D)
+    int ncomp;            // compute wavefronts
+};
+
+template <typename V, typename S, int R, int PPT, bool HAS_V, bool HAS_SRC>
+__global__ void __launch_bounds__(kRingMaxThreads)
+stencil_ring_kernel(const V* __restrict__ in, V* __restrict__ out,
+                    const S* __restrict__ vt, const V* __restrict__ src,
+                    S beta, CfdCoefs<S> cf, RingGeom rg, PhaseArgs<S> ph) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    V* ring = reinterpret_cast<V*>(smem_raw);
+    const CfdGeom& g = rg.g;
+    const int D = rg.nload;
+    const int P = R + D + 1;
+    const int n2 = g.n2;
+    const int tiles_per_band = g.nyt * g.nxc;
+    const int b = xcd_swizzle(blockIdx.x, gridDim.x);
+    const int band = b / tiles_per_band;
+    const int rem = b - band * tiles_per_band;
+    const int xc = rem / g.nyt;
+    const int yt = rem - xc * g.nyt;
+    const int y0 = yt * g.ty;
+    const int tye = min(g.ty, g.n1 - y0);
+    const int x0 = xc * g.lx;
+    const int x1 = min(x0 + g.lx, g.n0);
+    const size_t plane = (size_t)g.n1 * n2;
+    const size_t bstride = plane * g.n0;
+    const V* __restrict__ inb = in + band * bstride;
+    const int slot = (g.ty + 2 * R) * n2;   // elements per ring slot
+    const int wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const int xend = x1 + R;                 // planes [x0, xend) go through LDS
+
+    if (wave >= rg.ncomp) {
+        // ---------------------------------------------------- loader --
+        const int w = wave - rg.ncomp;
+        constexpr int EPL = kDmaBytes / (int)sizeof(V);   // elements per lane
+        constexpr int CH = 64 * EPL;                       // elements per op
+        // three row segments of the tile: rows below 0, inside, above n1
+        const int ylo = y0 - R, yhi = y0 + tye + R;        // [ylo, yhi)
+        int seg_g[3], seg_l[3], seg_n[3];
+        {
+            int a0 = ylo, a1 = min(0, yhi);                // wrapped low rows
+            int b0 = max(ylo, 0), b1 = min(yhi, g.n1);     // interior rows
+            int c0 = max(ylo, g.n1), c1 = yhi;             // wrapped high rows
+            seg_n[0] = (g.pbc1 && a1 > a0) ? (a1 - a0) * n2 : 0;
+            seg_g[0] = (a0 + g.n1) * n2;
+            seg_l[0] = 0;
+            seg_n[1] = (b1 - b0) * n2;
+            seg_g[1] = b0 * n2;
+            seg_l[1] = (b0 - ylo) * n2;
+            seg_n[2] = (g.pbc1 && c1 > c0) ? (c1 - c0) * n2 : 0;
+            seg_g[2] = (c0 - g.n1) * n2;
+            seg_l[2] = (c0 - ylo) * n2;
+        }
+        auto issue = [&](int xp) {
+            int xs = xp;
+            if (xs >= g.n0) {
+                if (!g.pbc0) return;
+                xs -= g.n0;
+            }
+            const V* src_pl = inb + (size_t)xs * plane;
+            V* dst = ring + ((xp - x0) % P) * slot;
+            for (int sgi = 0; sgi < 3; sgi++) {
+                const int n = seg_n[sgi];
+                for (int off = 0; off < n; off += CH) {
+                    const int e = off + lane * EPL;
+                    if (e < n)
+                        __builtin_amdgcn_global_load_lds(
+                            (const void*)(src_pl + seg_g[sgi] + e),
+                            (__attribute__((address_space(3))) void*)(dst + seg_l[sgi] + off),
+                            kDmaBytes, 0, 0);
+                }
+            }
+        };
+        for (int xp = x0 + w; xp < min(x0 + R + D, xend); xp += D) issue(xp);
+        for (int x = x0; x < x1; x++) {
+            // plane x+R (queue refill) is first read now; the planes x of
+            // the first R iterations were issued in the prologue and are
+            // first read now as well
+            if (((x + R - x0) % D) == w || (x - x0 < R && ((x - x0) % D) == w))
+                wait_vmcnt0();
+            ring_barrier();
+            const int xp = x + R + D;
+            if (xp < xend && ((xp - x0) % D) == w) issue(xp);
+        }
+        return;
+    }
+
+    // ------------------------------------------------------ compute --
+    V* __restrict__ outb = out + band * bstride;
+    const V* __restrict__ srcb = HAS_SRC ? src + band * bstride : nullptr;
+    const int NT = rg.ncomp * 64;
+    const int tile_pts = tye * n2;
+    int yl[PPT], zz[PPT], yz[PPT];
+    bool act[PPT];
+#pragma unroll
+    for (int k = 0; k < PPT; k++) {
+        int p = threadIdx.x + k * NT;
+        act[k] = p < tile_pts;
+        int pp = act[k] ? p : 0;
+        yl[k] = pp / n2;
+        zz[k] = pp - yl[k] * n2;
+        yz[k] = (y0 + yl[k]) * n2 + zz[k];
+    }
+    // y-ghost rows of the slot: R below (local rows 0..R-1) and R above
+    // (local rows R+tye..); kind per row offset j: 0 interior, 1 low wrap,
+    // 2 high wrap, 3 zero
+    auto row_kind = [&](int lrow) {
+        int y = y0 - R + lrow;
+        if (y >= 0 && y < g.n1) return 0;
+        if (!g.pbc1) return 3;
+        return y < 0 ? 1 : 2;
+    };
+    auto slot_val = [&](const V* sl, int lrow, int z) {
+        int kind = row_kind(lrow);
+        if (kind == 0) return sl[lrow * n2 + z];
+        if (kind == 3) return vzero<V>();
+        V v = sl[lrow * n2 + z];
+        return kind == 1 ? mul_phase(v, ph.re[1][1], ph.im[1][1])
+                         : mul_phase(v, ph.re[1][0], ph.im[1][0]);
+    };
+
+    V q[PPT][2 * R + 1];
+#pragma unroll
+    for (int j = 0; j < 2 * R; j++)
+#pragma unroll
+        for (int k = 0; k < PPT; k++)
+            q[k][j] = act[k] ? load_x(inb, x0 - R + j, yz[k], g, ph) : vzero<V>();
+    S nv[PPT];
+    V ns[PPT];
+#pragma unroll
+    for (int k = 0; k < PPT; k++) {
+        if (HAS_V) nv[k] = act[k] ? vt[(size_t)x0 * plane + yz[k]] : (S)0;
+        if (HAS_SRC) ns[k] = act[k] ? srcb[(size_t)x0 * plane + yz[k]] : vzero<V>();
+    }
+
+    for (int x = x0; x < x1; x++) {
+        ring_barrier();
+        // queue refill: plane x + R (wrapped with phase, or zero)
+        {
+            const int xp = x + R;
+            const V* sl = ring + ((xp - x0) % P) * slot;
+            const bool wrap = xp >= g.n0;
+            const bool zero = wrap && !g.pbc0;
+#pragma unroll
+            for (int k = 0; k < PPT; k++) {
+                V v = sl[(yl[k] + R) * n2 + zz[k]];
+                if (zero) v = vzero<V>();
+                else if (wrap) v = mul_phase(v, ph.re[0][0], ph.im[0][0]);
+                q[k][2 * R] = act[k] ? v : vzero<V>();
+            }
+        }
+        S cv[PPT];
+        V cs[PPT];
+#pragma unroll
+        for (int k = 0; k < PPT; k++) {
+            cv[k] = HAS_V ? nv[k] : (S)0;
+            cs[k] = HAS_SRC ? ns[k] : vzero<V>();
+        }
+        if (x + 1 < x1) {
+#pragma unroll
+            for (int k = 0; k < PPT; k++) {
+                if (HAS_V) nv[k] = act[k] ? vt[(size_t)(x + 1) * plane + yz[k]] : (S)0;
+                if (HAS_SRC) ns[k] = act[k] ? srcb[(size_t)(x + 1) * plane + yz[k]] : vzero<V>();
+            }
+        }
+        const V* sl = ring + ((x - x0) % P) * slot;
+#pragma unroll
+        for (int k = 0; k < PPT; k++) {
+            if (!act[k]) continue;
+            const int row = yl[k] + R;
+            const int z = zz[k];
+            V acc = vscale(q[k][R], cf.c0);
+#pragma unroll
+            for (int j = 0; j < R; j++) {
+                acc = fma_rs(cf.cm[0][j], q[k][R - 1 - j], acc);
+                acc = fma_rs(cf.cp[0][j], q[k][R + 1 + j], acc);
+            }
+#pragma unroll
+            for (int j = 0; j < R; j++) {
+                acc = fma_rs(cf.cm[1][j], slot_val(sl, row - 1 - j, z), acc);
+                acc = fma_rs(cf.cp[1][j], slot_val(sl, row + 1 + j, z), acc);
+            }
+            const V* brow = sl + row * n2;
+#pragma unroll
+            for (int j = 0; j < R; j++) {
+                int zm = z - 1 - j;
+                V vm;
+                if (zm >= 0) vm = brow[zm];
+                else if (g.pbc2) vm = mul_phase(brow[zm + n2], ph.re[2][1], ph.im[2][1]);
+                else vm = vzero<V>();
+                int zp = z + 1 + j;
+                V vp;
+                if (zp < n2) vp = brow[zp];
+                else if (g.pbc2) vp = mul_phase(brow[zp - n2], ph.re[2][0], ph.im[2][0]);
+                else vp = vzero<V>();
+                acc = fma_rs(cf.cm[2][j], vm, acc);
+                acc = fma_rs(cf.cp[2][j], vp, acc);
+            }
+            if (HAS_V) acc = fma_rs(cv[k], q[k][R], acc);
+            if (HAS_SRC) acc = fma_rs(beta, cs[k], acc);
+            outb[(size_t)x * plane + yz[k]] = acc;
         }
 #pragma unroll
         for (int k = 0; k < PPT; k++)
@@ -280,6 +578,7 @@ struct gh_stencil {
     std::vector<double> coefs;
     std::vector<int64_t> offs;
     int ty_req = 0, lx_req = 0;
+    bool reg_only = false;   // GPAW_AMD_STENCIL=reg selects the register kernel
 };
 
 using namespace gh;
@@ -317,7 +616,7 @@ TileChoice choose_tile(const gh_stencil* h, int vsize, int ppt) {
         for (int c = 1; c <= n1; c++) {
             size_t lds = 2 * (size_t)(c + 2 * R) * n2 * vsize;
             int thr = (c * n2 + ppt - 1) / ppt;
-            if (lds > lds_cap || thr > 1024) break;
+            if (lds > lds_cap || thr > kCfdMaxThreads) break;
             if (n1 % c == 0 || c == n1) best = c;
         }
         ty = best;
@@ -326,7 +625,7 @@ TileChoice choose_tile(const gh_stencil* h, int vsize, int ppt) {
             for (int c = n1; c >= 1; c--) {
                 size_t lds = 2 * (size_t)(c + 2 * R) * n2 * vsize;
                 int thr = (c * n2 + ppt - 1) / ppt;
-                if (lds <= lds_cap && thr <= 1024) { ty = c; break; }
+                if (lds <= lds_cap && thr <= kCfdMaxThreads) { ty = c; break; }
             }
         }
     }
@@ -345,7 +644,7 @@ int launch_cfd_r(const gh_stencil* h, const void* in, void* out,
                  const void* src, double beta, double alpha, double gamma,
                  hipStream_t st) {
     TileChoice t = choose_tile(h, sizeof(V), PPT);
-    GH_REQUIRE(t.threads <= 1024 && t.lds <= 64 * 1024, GH_ERR_UNSUPPORTED,
+    GH_REQUIRE(t.threads <= kCfdMaxThreads && t.lds <= 64 * 1024, GH_ERR_UNSUPPORTED,
                "stencil: grid rows of %lld points do not fit one block",
                (long long)h->n[2]);
     CfdGeom g;
@@ -367,26 +666,120 @@ int launch_cfd_r(const gh_stencil* h, const void* in, void* out,
     dim3 grid((unsigned)nblocks), block(t.threads);
     const S* vtp = (const S*)vt;
     const V* srcp = (const V*)src;
-    if (vt && src)
-        hipLaunchKernelGGL((stencil_cfd_kernel<V, S, R, PPT, true, true>), grid, block, t.lds, st,
-                           (const V*)in, (V*)out, vtp, srcp, (S)beta, cf, g, ph);
-    else if (vt)
-        hipLaunchKernelGGL((stencil_cfd_kernel<V, S, R, PPT, true, false>), grid, block, t.lds, st,
-                           (const V*)in, (V*)out, vtp, srcp, (S)beta, cf, g, ph);
-    else if (src)
-        hipLaunchKernelGGL((stencil_cfd_kernel<V, S, R, PPT, false, true>), grid, block, t.lds, st,
-                           (const V*)in, (V*)out, vtp, srcp, (S)beta, cf, g, ph);
-    else
-        hipLaunchKernelGGL((stencil_cfd_kernel<V, S, R, PPT, false, false>), grid, block, t.lds, st,
-                           (const V*)in, (V*)out, vtp, srcp, (S)beta, cf, g, ph);
+    const bool gs1 = 2 * R * g.n2 <= t.threads;
+#define GH_LAUNCH_CFD(GS, HV, HS)                                              \
+    hipLaunchKernelGGL((stencil_cfd_kernel<V, S, R, PPT, GS, HV, HS>), grid,   \
+                       block, t.lds, st, (const V*)in, (V*)out, vtp, srcp,     \
+                       (S)beta, cf, g, ph)
+    if (gs1) {
+        if (vt && src) GH_LAUNCH_CFD(1, true, true);
+        else if (vt) GH_LAUNCH_CFD(1, true, false);
+        else if (src) GH_LAUNCH_CFD(1, false, true);
+        else GH_LAUNCH_CFD(1, false, false);
+    } else {
+        if (vt && src) GH_LAUNCH_CFD(4, true, true);
+        else if (vt) GH_LAUNCH_CFD(4, true, false);
+        else if (src) GH_LAUNCH_CFD(4, false, true);
+        else GH_LAUNCH_CFD(4, false, false);
+    }
+#undef GH_LAUNCH_CFD
     GH_CHECK_LAUNCH();
     return GH_OK;
 }
+
+// LDS-ring launch: returns GH_ERR_UNSUPPORTED when the shape does not fit
+// (rows not 16-byte multiples, or no tile fits the LDS), so the caller can
+// fall back to the register kernel.
+constexpr size_t kLdsBytes = 160 * 1024;
+
+template <typename V, typename S, int R, int PPT>
+int launch_ring_r(const gh_stencil* h, const void* in, void* out,
+                  int64_t nbands, const double* phase_cd, const void* vt,
+                  const void* src, double beta, double alpha, double gamma,
+                  hipStream_t st) {
+    const int n0 = (int)h->n[0], n1 = (int)h->n[1], n2 = (int)h->n[2];
+    const int vs = (int)sizeof(V);
+    if ((n2 * vs) % kDmaBytes != 0) return GH_ERR_UNSUPPORTED;
+    int ty = 0, D = 0;
+    size_t lds = 0;
+    for (int d = 3; d >= 2 && ty < 8; d--) {
+        const int P = R + d + 1;
+        int best = 0;
+        for (int c = 1; c <= n1; c++) {
+            size_t l = (size_t)P * (c + 2 * R) * n2 * vs;
+            int thr = (c * n2 + PPT - 1) / PPT;
+            int ncomp = (thr + 63) / 64;
+            if (l > kLdsBytes - 1024 || (ncomp + d) * 64 > kRingMaxThreads) break;
+            if (n1 % c == 0 || c == n1 || best == 0) best = c;
+        }
+        if (h->ty_req > 0 && h->ty_req <= best) best = h->ty_req;
+        if (best > ty) {
+            ty = best;
+            D = d;
+            lds = (size_t)P * (best + 2 * R) * n2 * vs;
+        }
+    }
+    if (ty == 0) return GH_ERR_UNSUPPORTED;
+    RingGeom rg;
+    CfdGeom& g = rg.g;
+    g.n0 = n0; g.n1 = n1; g.n2 = n2;
+    g.pbc0 = h->pbc[0]; g.pbc1 = h->pbc[1]; g.pbc2 = h->pbc[2];
+    g.ty = ty;
+    g.lx = h->lx_req > 0 ? std::min(h->lx_req, n0) : n0;
+    g.nyt = (n1 + ty - 1) / ty;
+    g.nxc = (n0 + g.lx - 1) / g.lx;
+    rg.nload = D;
+    rg.ncomp = ((ty * n2 + PPT - 1) / PPT + 63) / 64;
+    CfdCoefs<S> cf;
+    cf.c0 = (S)(alpha * h->c0 + gamma);
+    for (int c = 0; c < 3; c++)
+        for (int j = 0; j < kMaxR; j++) {
+            cf.cm[c][j] = (S)(alpha * h->cm[c][j]);
+            cf.cp[c][j] = (S)(alpha * h->cp[c][j]);
+        }
+    PhaseArgs<S> ph = phase_args<S>(phase_cd);
+    int64_t nblocks = nbands * (int64_t)g.nyt * g.nxc;
+    GH_REQUIRE(nblocks < (1ll << 31), GH_ERR_SHAPE, "stencil: too many blocks");
+    dim3 grid((unsigned)nblocks), block((rg.ncomp + rg.nload) * 64);
+    const S* vtp = (const S*)vt;
+    const V* srcp = (const V*)src;
+#define GH_LAUNCH_RING(HV, HS)                                                 \
+    do {                                                                       \
+        auto kfn = stencil_ring_kernel<V, S, R, PPT, HV, HS>;                  \
+        GH_CHECK_HIP(hipFuncSetAttribute((const void*)kfn,                     \
+            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));            \
+        hipLaunchKernelGGL(kfn, grid, block, lds, st, (const V*)in, (V*)out,   \
+                           vtp, srcp, (S)beta, cf, rg, ph);                    \
+    } while (0)
+    if (vt && src) GH_LAUNCH_RING(true, true);
+    else if (vt) GH_LAUNCH_RING(true, false);
+    else if (src) GH_LAUNCH_RING(false, true);
+    else GH_LAUNCH_RING(false, false);
+#undef GH_LAUNCH_RING
+    GH_CHECK_LAUNCH();
+    return GH_OK;
+}
+
+// points per compute thread of the ring kernel
+template <typename V> struct RingPPT { static constexpr int value = 2; };
+template <> struct RingPPT<cplx<double>> { static constexpr int value = 1; };
+template <> struct RingPPT<float> { static constexpr int value = 2; };
+template <> struct RingPPT<cplx<float>> { static constexpr int value = 1; };
 
 template <typename V, typename S, int PPT>
 int launch_cfd(const gh_stencil* h, const void* in, void* out, int64_t nb,
                const double* ph, const void* vt, const void* src, double beta,
                double al, double ga, hipStream_t st) {
+    if (!h->reg_only) {
+        constexpr int RP = RingPPT<V>::value;
+        int rc = GH_ERR_UNSUPPORTED;
+        switch (h->R) {
+            case 1: rc = launch_ring_r<V, S, 1, RP>(h, in, out, nb, ph, vt, src, beta, al, ga, st); break;
+            case 2: rc = launch_ring_r<V, S, 2, RP>(h, in, out, nb, ph, vt, src, beta, al, ga, st); break;
+            case 3: rc = launch_ring_r<V, S, 3, RP>(h, in, out, nb, ph, vt, src, beta, al, ga, st); break;
+        }
+        if (rc != GH_ERR_UNSUPPORTED) return rc;
+    }
     switch (h->R) {
         case 1: return launch_cfd_r<V, S, 1, PPT>(h, in, out, nb, ph, vt, src, beta, al, ga, st);
         case 2: return launch_cfd_r<V, S, 2, PPT>(h, in, out, nb, ph, vt, src, beta, al, ga, st);
@@ -460,6 +853,10 @@ int gh_stencil_create(gh_stencil** out, int ncoefs, const double* coefs,
         h->pbc[c] = pbc_c[c] ? 1 : 0;
     }
     h->coefs.assign(coefs, coefs + ncoefs);
+    {
+        const char* e = getenv("GPAW_AMD_STENCIL");
+        h->reg_only = e && std::string(e) == "reg";
+    }
     h->offs.assign(offsets_pc, offsets_pc + 3 * ncoefs);
     h->R = 0;
     h->cfd = true;
@@ -538,10 +935,10 @@ int stencil_apply_scaled(const gh_stencil* h, const void* in_dev, void* out_dev,
     if (nbands == 0) return GH_OK;
     if (h->cfd) {
         switch (h->dtype) {
-            case GH_F64: return launch_cfd<double, double, 2>(h, in_dev, out_dev, nbands, phase_cd, vt_dev, src_dev, beta, al, ga, st);
-            case GH_C128: return launch_cfd<cplx<double>, double, 1>(h, in_dev, out_dev, nbands, phase_cd, vt_dev, src_dev, beta, al, ga, st);
-            case GH_F32: return launch_cfd<float, float, 4>(h, in_dev, out_dev, nbands, phase_cd, vt_dev, src_dev, beta, al, ga, st);
-            case GH_C64: return launch_cfd<cplx<float>, float, 2>(h, in_dev, out_dev, nbands, phase_cd, vt_dev, src_dev, beta, al, ga, st);
+            case GH_F64: return launch_cfd<double, double, 4>(h, in_dev, out_dev, nbands, phase_cd, vt_dev, src_dev, beta, al, ga, st);
+            case GH_C128: return launch_cfd<cplx<double>, double, 2>(h, in_dev, out_dev, nbands, phase_cd, vt_dev, src_dev, beta, al, ga, st);
+            case GH_F32: return launch_cfd<float, float, 8>(h, in_dev, out_dev, nbands, phase_cd, vt_dev, src_dev, beta, al, ga, st);
+            case GH_C64: return launch_cfd<cplx<float>, float, 4>(h, in_dev, out_dev, nbands, phase_cd, vt_dev, src_dev, beta, al, ga, st);
         }
     } else {
         switch (h->dtype) {

@@ -86,6 +86,39 @@ def test_laplace_vs_oracle(N_c, h, pbc, n, dtype, k_c):
     assert relerr(host(out), ref) < tol
 
 
+@pytest.mark.parametrize('kernel', ['ring', 'reg'])
+@pytest.mark.parametrize('N_c,pbc,dtype,k_c',
+                         [((24, 20, 28), True, np.float64, None),
+                          ((24, 20, 28), True, np.complex128, (0.25, 0, 0.1)),
+                          ((16, 17, 18), (False, False, True), np.float64,
+                           None),
+                          ((18, 14, 16), True, np.float32, None)])
+def test_stencil_kernel_variants(monkeypatch, kernel, N_c, pbc, dtype, k_c):
+    """Both stencil kernels (LDS-ring DMA and register-queue) on the same
+    inputs, with partial last tiles (odd rows per tile) and short x-runs."""
+    if kernel == 'reg':
+        monkeypatch.setenv('GPAW_AMD_STENCIL', 'reg')
+    else:
+        monkeypatch.delenv('GPAW_AMD_STENCIL', raising=False)
+    rng = np.random.default_rng(12)
+    gd = GridDescriptor(N_c, np.array(N_c) * 0.3, pbc)
+    ogd = og.Grid(N_c, np.array(N_c) * 0.3, pbc)
+    a = rand(rng, (3,) + tuple(gd.n_c), dtype)
+    vt = rng.standard_normal(tuple(gd.n_c)).astype(
+        np.float32 if dtype == np.float32 else np.float64)
+    ph = None if k_c is None else ogd.phase_cd(k_c)
+    c, o = ost.laplace_stencil(ogd.h_cv, -0.5, 3)
+    wide = np.complex128 if np.dtype(dtype).kind == 'c' else np.float64
+    ref = ost.fd_apply(c, o, a.astype(wide), ogd.pbc_c, ph) + vt * a
+    tol = 2e-5 if dtype == np.float32 else 1e-12
+    for rows, planes in ((0, 0), (5, 7), (3, 4)):
+        op = Laplace(gd, -0.5, 3, dtype)
+        op.set_tile(rows, planes)
+        out = torch.empty_like(dev(a))
+        op.apply(dev(a), out, ph, vt_g=dev(vt))
+        assert relerr(host(out), ref) < tol, (rows, planes)
+
+
 def test_laplace_fused_potential_and_source():
     rng = np.random.default_rng(2)
     N_c = (24, 20, 28)
