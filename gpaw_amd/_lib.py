@@ -31,7 +31,7 @@ _SIGNATURES = {
     'gh_stencil_destroy': (_c.c_int, [P]),
     'gh_stencil_diagonal': (D, [P]),
     'gh_stencil_apply': (_c.c_int, [P, P, P, I64, PD, P, P, D, P]),
-    'gh_stencil_set_tile': (_c.c_int, [P, _c.c_int, _c.c_int]),
+    'gh_stencil_set_tile': (_c.c_int, [P, _c.c_int, _c.c_int, _c.c_int]),
     'gh_transformer_create': (_c.c_int, [_c.POINTER(P), PI64, PI64, _c.c_int,
                                          PI64, PI64, PI32, _c.c_int,
                                          _c.c_int]),

@@ -81,6 +81,17 @@ __device__ __forceinline__ float vscale(float a, float s) { return a * s; }
 template <typename S>
 __device__ __forceinline__ cplx<S> vscale(cplx<S> a, S s) { return {a.re * s, a.im * s}; }
 
+// Streaming (non-temporal) store: the line is not kept in L2, for outputs
+// that are written once and not read again by the same kernel.
+__device__ __forceinline__ void store_nt(double* p, double v) { __builtin_nontemporal_store(v, p); }
+__device__ __forceinline__ void store_nt(float* p, float v) { __builtin_nontemporal_store(v, p); }
+template <typename S>
+__device__ __forceinline__ void store_nt(cplx<S>* p, cplx<S> v) {
+    typedef S pair_t __attribute__((ext_vector_type(2)));
+    pair_t w = {v.re, v.im};
+    __builtin_nontemporal_store(w, reinterpret_cast<pair_t*>(p));
+}
+
 // Host-side Bloch phases: phase_cd[c][d] as (re, im); identity when absent.
 struct Phases {
     double re[3][2];

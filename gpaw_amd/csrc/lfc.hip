@@ -16,12 +16,13 @@
 // group l / nm, so each lane keeps one accumulator per band and a wave reads
 // whole A rows with one coalesced load; partial sums are reduced through LDS
 // in a fixed order (deterministic).
-// add (the transposed product scattered back): atoms whose spheres share a
-// grid point get different colours (greedy colouring on the host); one
-// launch per colour, one block per (atom, band tile) with the atom's
-// coefficients staged in LDS.  Atoms of one colour touch disjoint points, so
-// every read-modify-write is race free without atomics, and the order of
-// the contributions to a point is fixed (deterministic).
+// add (the transposed product scattered back): add units (an atom's
+// entries, split into layers where its sphere overlaps its own periodic
+// image) that share a grid point get different colours (greedy colouring on
+// the host); one launch per colour, one wavefront per (unit, entry chunk,
+// band tile).  Units of one colour touch disjoint points, so every
+// read-modify-write is race free without atomics, and the order of the
+// contributions to a point is fixed (deterministic).
 #include <algorithm>
 #include <cmath>
 #include <numeric>
@@ -32,17 +33,22 @@
 namespace gh {
 
 constexpr int kLfcMaxNm = 64;   // functions per atom supported
-constexpr int kIntBT = 4;       // bands per integrate wavefront
-constexpr int kIntMC = 8;       // functions per integrate pass
-constexpr int kAddBT = 8;       // bands per add wavefront
+constexpr int kIntChunk = 256;  // entries staged per integrate step
+constexpr int kAddBT = 4;       // bands per add wavefront
 constexpr int kAddChunk = 512;  // entries per add wavefront
 
-// One wavefront per (atom, band tile, function chunk): lane = entry, so the
-// gathers of a[x, G_e] follow the sphere's z-runs (coalesced) and each lane
-// reads its A row contiguously; 32 partial sums per lane are reduced across
-// the wavefront once at the end (fixed butterfly order).
+template <typename V> struct IntBT { static constexpr int value = 16; };
+template <> struct IntBT<cplx<double>> { static constexpr int value = 8; };
+
+// integrate: one 256-thread block per (atom, band tile).  Per chunk of 256
+// entries every thread gathers a[x, G_e] for its entry and the tile's bands
+// (coalesced along the sphere's z-runs; the next chunk is prefetched into
+// registers) and stages them in LDS; then lane (g, m) = (entry group,
+// function) accumulates sum_e A[e, m] a[x, e] for all bands of the tile,
+// reading whole A rows with coalesced loads.  Group partial sums are
+// reduced through LDS in a fixed order (deterministic).
 template <typename V, typename S>
-__global__ void __launch_bounds__(64)
+__global__ void __launch_bounds__(256)
 lfc_integrate_kernel(const V* __restrict__ a, V* __restrict__ c,
                      const int64_t* __restrict__ entry_ptr,
                      const int32_t* __restrict__ M_ptr,
@@ -51,72 +57,92 @@ lfc_integrate_kernel(const V* __restrict__ a, V* __restrict__ c,
                      const int32_t* __restrict__ w_e,
                      const S* __restrict__ A, const S* __restrict__ phase_w,
                      int64_t ngpts, int Mmax, int nbands, S dv) {
+    constexpr int BT = IntBT<V>::value;
+    __shared__ V T[BT][kIntChunk];
     const int atom = blockIdx.x;
-    const int x0 = blockIdx.y * kIntBT;
+    const int x0 = blockIdx.y * BT;
     const int M0 = M_ptr[atom];
     const int nm = M_ptr[atom + 1] - M0;
-    const int m0 = blockIdx.z * kIntMC;
-    if (m0 >= nm) return;
-    const int mc = min(kIntMC, nm - m0);
+    if (nm == 0) return;
     const int64_t e0 = entry_ptr[atom];
     const int64_t e1 = entry_ptr[atom + 1];
     const S* __restrict__ Aa = A + A_off[atom];
-    const int lane = threadIdx.x;
-    const int nb = min(kIntBT, nbands - x0);
+    const int t = threadIdx.x;
+    const int nb = min(BT, nbands - x0);
+    const int ngrp = kIntChunk / nm;
+    const int grp = t / nm;
+    const int m = t - grp * nm;
+    const bool live = grp < ngrp;
 
-    V acc[kIntBT][kIntMC];
+    V pf[BT];
+    auto gather = [&](int64_t ebase) {
+        const int64_t e = ebase + t;
+        if (e < e1) {
+            const int64_t G = G_e[e];
+            S pre = 1, pim = 0;
+            if (is_complex<V>::value && phase_w) {
+                int w = w_e[e];
+                if (w >= 0) { pre = phase_w[2 * w]; pim = phase_w[2 * w + 1]; }
+            }
 #pragma unroll
-    for (int b = 0; b < kIntBT; b++)
+            for (int b = 0; b < BT; b++)
+                pf[b] = b < nb ? mul_phase(a[(size_t)(x0 + b) * ngpts + G], pre, pim)
+                               : vzero<V>();
+        } else {
 #pragma unroll
-        for (int m = 0; m < kIntMC; m++) acc[b][m] = vzero<V>();
-    for (int64_t e = e0 + lane; e < e1; e += 64) {
-        const int64_t G = G_e[e];
-        S pre = 1, pim = 0;
-        if (is_complex<V>::value && phase_w) {
-            int w = w_e[e];
-            if (w >= 0) { pre = phase_w[2 * w]; pim = phase_w[2 * w + 1]; }
+            for (int b = 0; b < BT; b++) pf[b] = vzero<V>();
         }
-        V av[kIntBT];
+    };
+
+    V acc[BT];
 #pragma unroll
-        for (int b = 0; b < kIntBT; b++)
-            av[b] = b < nb ? mul_phase(a[(size_t)(x0 + b) * ngpts + G], pre, pim)
-                           : vzero<V>();
-        const S* __restrict__ Ae = Aa + (e - e0) * nm + m0;
+    for (int b = 0; b < BT; b++) acc[b] = vzero<V>();
+    gather(e0);
+    for (int64_t eb = e0; eb < e1; eb += kIntChunk) {
+        __syncthreads();
 #pragma unroll
-        for (int m = 0; m < kIntMC; m++) {
-            if (m < mc) {
-                const S Am = Ae[m];
+        for (int b = 0; b < BT; b++) T[b][t] = pf[b];
+        if (eb + kIntChunk < e1) gather(eb + kIntChunk);
+        __syncthreads();
+        const int n = (int)min<int64_t>(kIntChunk, e1 - eb);
+        if (live) {
+            for (int el = grp; el < n; el += ngrp) {
+                const S Am = Aa[(eb - e0 + el) * nm + m];
 #pragma unroll
-                for (int b = 0; b < kIntBT; b++) acc[b][m] = fma_rs(Am, av[b], acc[b][m]);
+                for (int b = 0; b < BT; b++) acc[b] = fma_rs(Am, T[b][el], acc[b]);
             }
         }
     }
+    // reduce over entry groups: R[g][b][m] reuses T
+    __syncthreads();
+    V* Rb = &T[0][0];
+    if (live) {
 #pragma unroll
-    for (int b = 0; b < kIntBT; b++)
-#pragma unroll
-        for (int m = 0; m < kIntMC; m++) {
-            V v = acc[b][m];
-#pragma unroll
-            for (int off = 32; off > 0; off >>= 1) {
-                if constexpr (is_complex<V>::value) {
-                    v.re += __shfl_xor(v.re, off);
-                    v.im += __shfl_xor(v.im, off);
-                } else {
-                    v += __shfl_xor(v, off);
-                }
-            }
-            if (lane == b * kIntMC + m && b < nb && m < mc)
-                c[(size_t)(x0 + b) * Mmax + M0 + m0 + m] = vscale(v, dv);
-        }
+        for (int b = 0; b < BT; b++) Rb[(grp * BT + b) * nm + m] = acc[b];
+    }
+    __syncthreads();
+    for (int u = t; u < BT * nm; u += blockDim.x) {
+        const int b = u / nm;
+        const int mm = u - b * nm;
+        V sum = vzero<V>();
+        for (int g = 0; g < ngrp; g++) sum = vadd(sum, Rb[(g * BT + b) * nm + mm]);
+        if (b < nb) c[(size_t)(x0 + b) * Mmax + M0 + mm] = vscale(sum, dv);
+    }
 }
 
-// One wavefront per (atom of the colour, entry chunk, band tile): lane =
-// entry; the atom's coefficients for the band tile are staged in LDS and
-// read as broadcasts.
-template <typename V, typename S>
+// add: one wavefront per (unit of the colour, entry chunk, band tile).  A
+// unit is a contiguous entry range of one atom in which every grid point
+// occurs once (an atom whose sphere overlaps its own periodic image has
+// several units).  lane = entry; the tile's coefficients c[x, M_a + m] are
+// wave-uniform (scalar registers when the function count is known at
+// compile time).
+template <typename V, typename S, int NM>
 __global__ void __launch_bounds__(64)
 lfc_add_kernel(const V* __restrict__ c, V* __restrict__ a,
-               const int32_t* __restrict__ atoms, int nchunks,
+               const int32_t* __restrict__ units, int nchunks,
+               const int32_t* __restrict__ unit_atom,
+               const int64_t* __restrict__ unit_beg,
+               const int64_t* __restrict__ unit_end,
                const int64_t* __restrict__ entry_ptr,
                const int32_t* __restrict__ M_ptr,
                const int64_t* __restrict__ A_off,
@@ -124,23 +150,17 @@ lfc_add_kernel(const V* __restrict__ c, V* __restrict__ a,
                const int32_t* __restrict__ w_e, const S* __restrict__ A,
                const S* __restrict__ phase_w, int64_t ngpts, int Mmax,
                int nbands) {
-    __shared__ V cl[kAddBT][kLfcMaxNm];
-    const int atom = atoms[blockIdx.x / nchunks];
+    const int unit = units[blockIdx.x / nchunks];
     const int chunk = blockIdx.x % nchunks;
+    const int atom = unit_atom[unit];
     const int x0 = blockIdx.y * kAddBT;
     const int64_t e0 = entry_ptr[atom];
-    const int64_t eb = e0 + (int64_t)chunk * kAddChunk;
-    const int64_t e1 = min(entry_ptr[atom + 1], eb + kAddChunk);
+    const int64_t eb = unit_beg[unit] + (int64_t)chunk * kAddChunk;
+    const int64_t e1 = min(unit_end[unit], eb + kAddChunk);
     if (eb >= e1) return;
     const int M0 = M_ptr[atom];
-    const int nm = M_ptr[atom + 1] - M0;
+    const int nm = NM > 0 ? NM : M_ptr[atom + 1] - M0;
     const int nb = min(kAddBT, nbands - x0);
-    for (int t = threadIdx.x; t < kAddBT * nm; t += 64) {
-        const int b = t / nm;
-        const int m = t - b * nm;
-        cl[b][m] = b < nb ? c[(size_t)(x0 + b) * Mmax + M0 + m] : vzero<V>();
-    }
-    __syncthreads();
     const S* __restrict__ Aa = A + A_off[atom];
     for (int64_t e = eb + threadIdx.x; e < e1; e += 64) {
         const int64_t G = G_e[e];
@@ -148,10 +168,22 @@ lfc_add_kernel(const V* __restrict__ c, V* __restrict__ a,
         V s[kAddBT];
 #pragma unroll
         for (int b = 0; b < kAddBT; b++) s[b] = vzero<V>();
-        for (int m = 0; m < nm; m++) {
-            const S Am = Ae[m];
 #pragma unroll
-            for (int b = 0; b < kAddBT; b++) s[b] = fma_rs(Am, cl[b][m], s[b]);
+        for (int m = 0; m < (NM > 0 ? NM : 1); m++) {
+            if (NM > 0) {
+                const S Am = Ae[m];
+#pragma unroll
+                for (int b = 0; b < kAddBT; b++)
+                    if (b < nb) s[b] = fma_rs(Am, c[(size_t)(x0 + b) * Mmax + M0 + m], s[b]);
+            }
+        }
+        if (NM == 0) {
+            for (int m = 0; m < nm; m++) {
+                const S Am = Ae[m];
+#pragma unroll
+                for (int b = 0; b < kAddBT; b++)
+                    if (b < nb) s[b] = fma_rs(Am, c[(size_t)(x0 + b) * Mmax + M0 + m], s[b]);
+            }
         }
         S pre = 1, pim = 0;
         if (is_complex<V>::value && phase_w) {
@@ -220,8 +252,12 @@ struct gh_lfc {
     void* A = nullptr;         // double or float
     void* phase_w = nullptr;   // (re, im) pairs, double or float
     int max_nm = 0;                     // most functions on one atom
-    int64_t max_entries = 0;            // most entries on one atom
-    int32_t* color_atoms = nullptr;     // atoms grouped by colour
+    int uniform_nm = -1;                // functions per atom if all equal
+    int64_t max_unit = 0;               // most entries in one add unit
+    int32_t* unit_atom = nullptr;       // add unit -> atom
+    int64_t* unit_beg = nullptr;        // add unit -> entry range
+    int64_t* unit_end = nullptr;
+    int32_t* color_units = nullptr;     // add units grouped by colour
     std::vector<int> color_ptr;         // host: colour c -> [ptr[c], ptr[c+1])
 };
 
@@ -241,7 +277,8 @@ int upload(T** dst, const T* src, size_t n) {
 void free_lfc(gh_lfc* h) {
     if (!h) return;
     void* ptrs[] = {h->entry_ptr, h->M_ptr, h->A_off, h->dH_off, h->G_e, h->w_e,
-                    h->A, h->phase_w, h->color_atoms};
+                    h->A, h->phase_w, h->unit_atom, h->unit_beg, h->unit_end,
+                    h->color_units};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     delete h;
@@ -266,7 +303,6 @@ int gh_lfc_create(gh_lfc** out, int64_t ngpts, int natoms,
     GH_REQUIRE(ne == 0 || (G_e && A), GH_ERR_ARG, "lfc_create: missing entries");
     // host-side validation and derived tables
     std::vector<int64_t> A_off(natoms + 1, 0), dH_off(natoms + 1, 0);
-    std::vector<int32_t> atom_of_e(ne);
     for (int a = 0; a < natoms; a++) {
         int nm = M_ptr[a + 1] - M_ptr[a];
         GH_REQUIRE(nm >= 0 && entry_ptr[a + 1] >= entry_ptr[a], GH_ERR_ARG,
@@ -279,44 +315,84 @@ int gh_lfc_create(gh_lfc** out, int64_t ngpts, int natoms,
             GH_REQUIRE(G_e[e] >= 0 && G_e[e] < ngpts, GH_ERR_ARG,
                        "lfc_create: grid index out of range");
             GH_REQUIRE(!w_e || w_e[e] < nw, GH_ERR_ARG, "lfc_create: image index out of range");
-            atom_of_e[e] = a;
+            GH_REQUIRE(!w_e || w_e[e] >= -1, GH_ERR_ARG, "lfc_create: bad image index");
         }
     }
-    // colour atoms so that atoms sharing a grid point differ in colour
+    // Order each atom's entries by layer: the k-th occurrence of a grid
+    // point within the atom (a sphere overlapping its own periodic image)
+    // goes to layer k.  A layer touches each point once and is one add unit.
+    std::vector<int64_t> perm(ne);
+    std::vector<int32_t> unit_atom;
+    std::vector<int64_t> unit_beg, unit_end;
+    for (int a = 0; a < natoms; a++) {
+        const int64_t e0 = entry_ptr[a], n = entry_ptr[a + 1] - e0;
+        if (n == 0) continue;
+        std::vector<int64_t> byG(n);
+        std::iota(byG.begin(), byG.end(), 0);
+        std::stable_sort(byG.begin(), byG.end(), [&](int64_t x, int64_t y) {
+            return G_e[e0 + x] < G_e[e0 + y];
+        });
+        std::vector<int> occ(n, 0);
+        for (int64_t i = 1; i < n; i++)
+            if (G_e[e0 + byG[i]] == G_e[e0 + byG[i - 1]]) occ[byG[i]] = occ[byG[i - 1]] + 1;
+        std::vector<int64_t> loc(n);
+        std::iota(loc.begin(), loc.end(), 0);
+        std::stable_sort(loc.begin(), loc.end(),
+                         [&](int64_t x, int64_t y) { return occ[x] < occ[y]; });
+        for (int64_t i = 0; i < n; i++) {
+            perm[e0 + i] = e0 + loc[i];
+            if (i == 0 || occ[loc[i]] != occ[loc[i - 1]]) {
+                if (i > 0) unit_end.push_back(e0 + i);
+                unit_atom.push_back(a);
+                unit_beg.push_back(e0 + i);
+            }
+        }
+        unit_end.push_back(e0 + n);
+    }
+    const int nunits = (int)unit_atom.size();
+    std::vector<int32_t> G2(ne), w2(ne, -1), unit_of_e(ne);
+    for (int64_t e = 0; e < ne; e++) {
+        G2[e] = G_e[perm[e]];
+        if (w_e) w2[e] = w_e[perm[e]];
+    }
+    for (int u = 0; u < nunits; u++)
+        for (int64_t e = unit_beg[u]; e < unit_end[u]; e++) unit_of_e[e] = u;
+    // colour the units so that units sharing a grid point differ in colour
     std::vector<int64_t> order(ne);
     std::iota(order.begin(), order.end(), 0);
     std::stable_sort(order.begin(), order.end(),
-                     [&](int64_t x, int64_t y) { return G_e[x] < G_e[y]; });
-    std::vector<std::vector<int>> nbr(natoms);
+                     [&](int64_t x, int64_t y) { return G2[x] < G2[y]; });
+    std::vector<std::vector<int>> nbr(nunits);
     int64_t nunique = 0;
     for (int64_t i = 0; i < ne;) {
         int64_t j = i;
-        while (j < ne && G_e[order[j]] == G_e[order[i]]) j++;
+        while (j < ne && G2[order[j]] == G2[order[i]]) j++;
         nunique++;
         for (int64_t p = i; p < j; p++)
             for (int64_t q = p + 1; q < j; q++) {
-                int aa = atom_of_e[order[p]], bb = atom_of_e[order[q]];
-                if (aa != bb) { nbr[aa].push_back(bb); nbr[bb].push_back(aa); }
+                int uu = unit_of_e[order[p]], vv = unit_of_e[order[q]];
+                if (uu != vv) { nbr[uu].push_back(vv); nbr[vv].push_back(uu); }
             }
         i = j;
     }
-    std::vector<int> color(natoms, -1);
+    std::vector<int> color(nunits, -1);
     int ncolors = 0;
-    for (int a = 0; a < natoms; a++) {
+    for (int u = 0; u < nunits; u++) {
         std::vector<char> used(ncolors + 1, 0);
-        for (int b : nbr[a])
-            if (color[b] >= 0) used[color[b]] = 1;
+        for (int v : nbr[u])
+            if (color[v] >= 0) used[color[v]] = 1;
         int c = 0;
         while (used[c]) c++;
-        color[a] = c;
+        color[u] = c;
         ncolors = std::max(ncolors, c + 1);
     }
-    std::vector<int32_t> color_atoms;
+    std::vector<int32_t> color_units;
     std::vector<int> color_ptr(1, 0);
     for (int c = 0; c < ncolors; c++) {
-        for (int a = 0; a < natoms; a++)
-            if (color[a] == c && M_ptr[a + 1] > M_ptr[a]) color_atoms.push_back(a);
-        color_ptr.push_back((int)color_atoms.size());
+        for (int u = 0; u < nunits; u++)
+            if (color[u] == c && M_ptr[unit_atom[u] + 1] > M_ptr[unit_atom[u]])
+                color_units.push_back(u);
+        color_ptr.push_back((int)color_units.size());
     }
 
     auto* h = new gh_lfc();
@@ -326,26 +402,40 @@ int gh_lfc_create(gh_lfc** out, int64_t ngpts, int natoms,
     h->nu = (int)nunique;
     h->color_ptr = color_ptr;
     for (int a = 0; a < natoms; a++) {
+        const int nma = M_ptr[a + 1] - M_ptr[a];
+        if (a == 0) h->uniform_nm = nma;
+        else if (h->uniform_nm != nma) h->uniform_nm = -1;
         h->max_nm = std::max(h->max_nm, M_ptr[a + 1] - M_ptr[a]);
-        h->max_entries = std::max(h->max_entries, entry_ptr[a + 1] - entry_ptr[a]);
     }
+    for (int u = 0; u < nunits; u++)
+        h->max_unit = std::max(h->max_unit, unit_end[u] - unit_beg[u]);
     h->ngpts = ngpts;
     h->nentries = ne;
     h->dv = dv;
+    // function values in the permuted entry order
+    std::vector<double> A2((size_t)A_off[natoms]);
+    for (int a = 0; a < natoms; a++) {
+        const int nm = M_ptr[a + 1] - M_ptr[a];
+        for (int64_t e = entry_ptr[a]; e < entry_ptr[a + 1]; e++)
+            std::copy(A + A_off[a] + (perm[e] - entry_ptr[a]) * nm,
+                      A + A_off[a] + (perm[e] - entry_ptr[a] + 1) * nm,
+                      A2.begin() + A_off[a] + (e - entry_ptr[a]) * nm);
+    }
     int rc;
 #define UP(dst, src, n) if ((rc = upload(dst, src, n)) != GH_OK) { free_lfc(h); return rc; }
     UP(&h->entry_ptr, entry_ptr, (size_t)natoms + 1);
     UP(&h->M_ptr, M_ptr, (size_t)natoms + 1);
     UP(&h->A_off, A_off.data(), A_off.size());
     UP(&h->dH_off, dH_off.data(), dH_off.size());
-    UP(&h->G_e, G_e, (size_t)ne);
-    UP(&h->color_atoms, color_atoms.data(), color_atoms.size());
-    std::vector<int32_t> w(ne, -1);
-    if (w_e) std::copy(w_e, w_e + ne, w.begin());
-    UP(&h->w_e, w.data(), w.size());
-    const size_t nA = (size_t)A_off[natoms];
+    UP(&h->G_e, G2.data(), (size_t)ne);
+    UP(&h->w_e, w2.data(), w2.size());
+    UP(&h->unit_atom, unit_atom.data(), unit_atom.size());
+    UP(&h->unit_beg, unit_beg.data(), unit_beg.size());
+    UP(&h->unit_end, unit_end.data(), unit_end.size());
+    UP(&h->color_units, color_units.data(), color_units.size());
+    const size_t nA = A2.size();
     if (single(dtype)) {
-        std::vector<float> Af(A, A + nA);
+        std::vector<float> Af(A2.begin(), A2.end());
         float* d = nullptr;
         UP(&d, Af.data(), nA);
         h->A = d;
@@ -357,7 +447,7 @@ int gh_lfc_create(gh_lfc** out, int64_t ngpts, int natoms,
         }
     } else {
         double* d = nullptr;
-        UP(&d, A, nA);
+        UP(&d, A2.data(), nA);
         h->A = d;
         if (phase_w && nw > 0) {
             double* dp = nullptr;
@@ -384,16 +474,16 @@ int gh_lfc_integrate(const gh_lfc* h, const void* a_dev, void* c_dev,
     // every (band, M) is written by the block of its atom, also for atoms
     // without grid points, so no clearing pass is needed
     hipStream_t st = (hipStream_t)stream;
-    dim3 grid(h->natoms, (unsigned)((nbands + kIntBT - 1) / kIntBT),
-              (unsigned)((h->max_nm + kIntMC - 1) / kIntMC));
+    const int bt = h->dtype == GH_C128 ? IntBT<cplx<double>>::value : IntBT<double>::value;
+    dim3 grid(h->natoms, (unsigned)((nbands + bt - 1) / bt));
     const bool ph = cplxd(h->dtype) && h->phase_w;
 #define ARGS(V, S) (const V*)a_dev, (V*)c_dev, h->entry_ptr, h->M_ptr, h->A_off, h->G_e, h->w_e, \
                    (const S*)h->A, ph ? (const S*)h->phase_w : nullptr, h->ngpts, h->Mmax, (int)nbands, (S)h->dv
     switch (h->dtype) {
-        case GH_F64: hipLaunchKernelGGL((lfc_integrate_kernel<double, double>), grid, dim3(64), 0, st, ARGS(double, double)); break;
-        case GH_C128: hipLaunchKernelGGL((lfc_integrate_kernel<cplx<double>, double>), grid, dim3(64), 0, st, ARGS(cplx<double>, double)); break;
-        case GH_F32: hipLaunchKernelGGL((lfc_integrate_kernel<float, float>), grid, dim3(64), 0, st, ARGS(float, float)); break;
-        case GH_C64: hipLaunchKernelGGL((lfc_integrate_kernel<cplx<float>, float>), grid, dim3(64), 0, st, ARGS(cplx<float>, float)); break;
+        case GH_F64: hipLaunchKernelGGL((lfc_integrate_kernel<double, double>), grid, dim3(256), 0, st, ARGS(double, double)); break;
+        case GH_C128: hipLaunchKernelGGL((lfc_integrate_kernel<cplx<double>, double>), grid, dim3(256), 0, st, ARGS(cplx<double>, double)); break;
+        case GH_F32: hipLaunchKernelGGL((lfc_integrate_kernel<float, float>), grid, dim3(256), 0, st, ARGS(float, float)); break;
+        case GH_C64: hipLaunchKernelGGL((lfc_integrate_kernel<cplx<float>, float>), grid, dim3(256), 0, st, ARGS(cplx<float>, float)); break;
     }
 #undef ARGS
     GH_CHECK_LAUNCH();
@@ -407,20 +497,28 @@ int gh_lfc_add(const gh_lfc* h, const void* c_dev, void* a_dev,
     hipStream_t st = (hipStream_t)stream;
     const bool ph = cplxd(h->dtype) && h->phase_w;
     const unsigned ntiles = (unsigned)((nbands + kAddBT - 1) / kAddBT);
+    const bool uniform8 = h->uniform_nm == 8;
     for (size_t col = 0; col + 1 < h->color_ptr.size(); col++) {
         const int n = h->color_ptr[col + 1] - h->color_ptr[col];
         if (n == 0) continue;
-        const int nchunks = (int)((h->max_entries + kAddChunk - 1) / kAddChunk);
+        const int nchunks = (int)((h->max_unit + kAddChunk - 1) / kAddChunk);
         dim3 grid(n * nchunks, ntiles);
-        const int32_t* atoms = h->color_atoms + h->color_ptr[col];
-#define ARGS(V, S) (const V*)c_dev, (V*)a_dev, atoms, nchunks, h->entry_ptr, h->M_ptr, h->A_off, \
+        const int32_t* units = h->color_units + h->color_ptr[col];
+#define ARGS(V, S) (const V*)c_dev, (V*)a_dev, units, nchunks, h->unit_atom, h->unit_beg, \
+                   h->unit_end, h->entry_ptr, h->M_ptr, h->A_off, \
                    h->G_e, h->w_e, (const S*)h->A, ph ? (const S*)h->phase_w : nullptr, \
                    h->ngpts, h->Mmax, (int)nbands
         switch (h->dtype) {
-            case GH_F64: hipLaunchKernelGGL((lfc_add_kernel<double, double>), grid, dim3(64), 0, st, ARGS(double, double)); break;
-            case GH_C128: hipLaunchKernelGGL((lfc_add_kernel<cplx<double>, double>), grid, dim3(64), 0, st, ARGS(cplx<double>, double)); break;
-            case GH_F32: hipLaunchKernelGGL((lfc_add_kernel<float, float>), grid, dim3(64), 0, st, ARGS(float, float)); break;
-            case GH_C64: hipLaunchKernelGGL((lfc_add_kernel<cplx<float>, float>), grid, dim3(64), 0, st, ARGS(cplx<float>, float)); break;
+            case GH_F64:
+                if (uniform8) hipLaunchKernelGGL((lfc_add_kernel<double, double, 8>), grid, dim3(64), 0, st, ARGS(double, double));
+                else hipLaunchKernelGGL((lfc_add_kernel<double, double, 0>), grid, dim3(64), 0, st, ARGS(double, double));
+                break;
+            case GH_C128:
+                if (uniform8) hipLaunchKernelGGL((lfc_add_kernel<cplx<double>, double, 8>), grid, dim3(64), 0, st, ARGS(cplx<double>, double));
+                else hipLaunchKernelGGL((lfc_add_kernel<cplx<double>, double, 0>), grid, dim3(64), 0, st, ARGS(cplx<double>, double));
+                break;
+            case GH_F32: hipLaunchKernelGGL((lfc_add_kernel<float, float, 0>), grid, dim3(64), 0, st, ARGS(float, float)); break;
+            case GH_C64: hipLaunchKernelGGL((lfc_add_kernel<cplx<float>, float, 0>), grid, dim3(64), 0, st, ARGS(cplx<float>, float)); break;
         }
 #undef ARGS
         GH_CHECK_LAUNCH();

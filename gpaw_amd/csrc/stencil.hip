@@ -287,25 +287,55 @@ struct RingGeom {
     CfdGeom g;
     int nload;            // loader wavefronts (= prefetch depth D)
     int ncomp;            // compute wavefronts
+    int bgroup;           // bands per scheduling group
+    int nt_store;         // streaming stores of the output
+    int nbands;
+};
+
+// Block -> (band, x chunk, y tile).  Logical ids of one XCD are contiguous
+// (xcd_swizzle); within them bands come in groups of `bgroup`, all tiles of
+// a group being consecutive, so the blocks that run together on an XCD are
+// every tile of a few bands: the potential tile of a plane is then shared
+// in L2 by the group's bands and the ghost rows of a tile are the rows its
+// neighbour tile (same band, running alongside) has just brought in.
+__device__ __forceinline__ void ring_block_coords(const RingGeom& rg, int L,
+                                                  int& band, int& xc,
+                                                  int& yt) {
+    const int tiles = rg.g.nyt * rg.g.nxc;
+    const int per_group = rg.bgroup * tiles;
+    const int grp = L / per_group;
+    const int bg = min(rg.bgroup, rg.nbands - grp * rg.bgroup);
+    const int rem = L - grp * per_group;
+    const int tile = rem / bg;
+    band = grp * rg.bgroup + (rem - tile * bg);
+    xc = tile / rg.g.nyt;
+    yt = tile - xc * rg.g.nyt;
+}
+
+// LDS layout of the ring kernel (in elements of V): P psi slots of
+// (ty + 2R) rows, then D + 1 slots of ty rows for the potential (as S) and
+// D + 1 slots of ty rows for src, when present.
+struct RingLayout {
+    int psi_slot, aux_slot;     // elements per slot
+    int v_off, s_off;           // byte offsets of the potential / src rings
 };
 
 template <typename V, typename S, int R, int PPT, bool HAS_V, bool HAS_SRC>
 __global__ void __launch_bounds__(kRingMaxThreads)
 stencil_ring_kernel(const V* __restrict__ in, V* __restrict__ out,
                     const S* __restrict__ vt, const V* __restrict__ src,
-                    S beta, CfdCoefs<S> cf, RingGeom rg, PhaseArgs<S> ph) {
+                    S beta, CfdCoefs<S> cf, RingGeom rg, PhaseArgs<S> ph,
+                    RingLayout lay) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     V* ring = reinterpret_cast<V*>(smem_raw);
+    S* vring = reinterpret_cast<S*>(smem_raw + lay.v_off);
+    V* sring = reinterpret_cast<V*>(smem_raw + lay.s_off);
     const CfdGeom& g = rg.g;
     const int D = rg.nload;
     const int P = R + D + 1;
     const int n2 = g.n2;
-    const int tiles_per_band = g.nyt * g.nxc;
-    const int b = xcd_swizzle(blockIdx.x, gridDim.x);
-    const int band = b / tiles_per_band;
-    const int rem = b - band * tiles_per_band;
-    const int xc = rem / g.nyt;
-    const int yt = rem - xc * g.nyt;
+    int band, xc, yt;
+    ring_block_coords(rg, xcd_swizzle(blockIdx.x, gridDim.x), band, xc, yt);
     const int y0 = yt * g.ty;
     const int tye = min(g.ty, g.n1 - y0);
     const int x0 = xc * g.lx;
@@ -360,7 +390,43 @@ stencil_ring_kernel(const V* __restrict__ in, V* __restrict__ out,
                 }
             }
         };
-        for (int xp = x0 + w; xp < min(x0 + R + D, xend); xp += D) issue(xp);
+        // tile rows of plane xv of the potential and of src (no ghosts);
+        // plane xv is loaded by the owner of psi plane xv + R, together
+        // with it, into slot (xv - x0) % (D + 1)
+        const int aux_n = tye * n2;
+        auto issue_aux = [&](int xv) {
+            if (xv >= x1) return;
+            const int sl = (xv - x0) % (D + 1);
+            if (HAS_V) {
+                constexpr int EPLS = kDmaBytes / (int)sizeof(S);
+                const S* srcp = vt + (size_t)xv * plane + (size_t)y0 * n2;
+                S* dst = vring + sl * lay.aux_slot;
+                for (int off = 0; off < aux_n; off += 64 * EPLS) {
+                    const int e = off + lane * EPLS;
+                    if (e < aux_n)
+                        __builtin_amdgcn_global_load_lds(
+                            (const void*)(srcp + e),
+                            (__attribute__((address_space(3))) void*)(dst + off),
+                            kDmaBytes, 0, 0);
+                }
+            }
+            if (HAS_SRC) {
+                const V* srcp = src + band * bstride + (size_t)xv * plane + (size_t)y0 * n2;
+                V* dst = sring + sl * lay.aux_slot;
+                for (int off = 0; off < aux_n; off += CH) {
+                    const int e = off + lane * EPL;
+                    if (e < aux_n)
+                        __builtin_amdgcn_global_load_lds(
+                            (const void*)(srcp + e),
+                            (__attribute__((address_space(3))) void*)(dst + off),
+                            kDmaBytes, 0, 0);
+                }
+            }
+        };
+        for (int xp = x0 + w; xp < min(x0 + R + D, xend); xp += D) {
+            issue(xp);
+            if (xp - R >= x0) issue_aux(xp - R);
+        }
         for (int x = x0; x < x1; x++) {
             // plane x+R (queue refill) is first read now; the planes x of
             // the first R iterations were issued in the prologue and are
@@ -369,14 +435,16 @@ stencil_ring_kernel(const V* __restrict__ in, V* __restrict__ out,
                 wait_vmcnt0();
             ring_barrier();
             const int xp = x + R + D;
-            if (xp < xend && ((xp - x0) % D) == w) issue(xp);
+            if (xp < xend && ((xp - x0) % D) == w) {
+                issue(xp);
+                issue_aux(xp - R);
+            }
         }
         return;
     }
 
     // ------------------------------------------------------ compute --
     V* __restrict__ outb = out + band * bstride;
-    const V* __restrict__ srcb = HAS_SRC ? src + band * bstride : nullptr;
     const int NT = rg.ncomp * 64;
     const int tile_pts = tye * n2;
     int yl[PPT], zz[PPT], yz[PPT];
@@ -414,14 +482,6 @@ stencil_ring_kernel(const V* __restrict__ in, V* __restrict__ out,
 #pragma unroll
         for (int k = 0; k < PPT; k++)
             q[k][j] = act[k] ? load_x(inb, x0 - R + j, yz[k], g, ph) : vzero<V>();
-    S nv[PPT];
-    V ns[PPT];
-#pragma unroll
-    for (int k = 0; k < PPT; k++) {
-        if (HAS_V) nv[k] = act[k] ? vt[(size_t)x0 * plane + yz[k]] : (S)0;
-        if (HAS_SRC) ns[k] = act[k] ? srcb[(size_t)x0 * plane + yz[k]] : vzero<V>();
-    }
-
     for (int x = x0; x < x1; x++) {
         ring_barrier();
         // queue refill: plane x + R (wrapped with phase, or zero)
@@ -438,20 +498,7 @@ stencil_ring_kernel(const V* __restrict__ in, V* __restrict__ out,
                 q[k][2 * R] = act[k] ? v : vzero<V>();
             }
         }
-        S cv[PPT];
-        V cs[PPT];
-#pragma unroll
-        for (int k = 0; k < PPT; k++) {
-            cv[k] = HAS_V ? nv[k] : (S)0;
-            cs[k] = HAS_SRC ? ns[k] : vzero<V>();
-        }
-        if (x + 1 < x1) {
-#pragma unroll
-            for (int k = 0; k < PPT; k++) {
-                if (HAS_V) nv[k] = act[k] ? vt[(size_t)(x + 1) * plane + yz[k]] : (S)0;
-                if (HAS_SRC) ns[k] = act[k] ? srcb[(size_t)(x + 1) * plane + yz[k]] : vzero<V>();
-            }
-        }
+        const int aux = ((x - x0) % (D + 1)) * lay.aux_slot;
         const V* sl = ring + ((x - x0) % P) * slot;
 #pragma unroll
         for (int k = 0; k < PPT; k++) {
@@ -485,9 +532,11 @@ stencil_ring_kernel(const V* __restrict__ in, V* __restrict__ out,
                 acc = fma_rs(cf.cm[2][j], vm, acc);
                 acc = fma_rs(cf.cp[2][j], vp, acc);
             }
-            if (HAS_V) acc = fma_rs(cv[k], q[k][R], acc);
-            if (HAS_SRC) acc = fma_rs(beta, cs[k], acc);
-            outb[(size_t)x * plane + yz[k]] = acc;
+            const int own = yl[k] * n2 + z;
+            if (HAS_V) acc = fma_rs(vring[aux + own], q[k][R], acc);
+            if (HAS_SRC) acc = fma_rs(beta, sring[aux + own], acc);
+            if (rg.nt_store) store_nt(outb + (size_t)x * plane + yz[k], acc);
+            else outb[(size_t)x * plane + yz[k]] = acc;
         }
 #pragma unroll
         for (int k = 0; k < PPT; k++)
@@ -577,7 +626,8 @@ struct gh_stencil {
     // generic representation
     std::vector<double> coefs;
     std::vector<int64_t> offs;
-    int ty_req = 0, lx_req = 0;
+    int ty_req = 0, lx_req = 0, depth_req = 0;
+    int bgroup_req = 0;      // bands per scheduling group (GPAW_AMD_BGROUP)
     bool reg_only = false;   // GPAW_AMD_STENCIL=reg selects the register kernel
 };
 
@@ -700,26 +750,48 @@ int launch_ring_r(const gh_stencil* h, const void* in, void* out,
     const int n0 = (int)h->n[0], n1 = (int)h->n[1], n2 = (int)h->n[2];
     const int vs = (int)sizeof(V);
     if ((n2 * vs) % kDmaBytes != 0) return GH_ERR_UNSUPPORTED;
-    int ty = 0, D = 0;
-    size_t lds = 0;
-    for (int d = 3; d >= 2 && ty < 8; d--) {
-        const int P = R + d + 1;
+    if (vt && (n2 * (int)sizeof(S)) % kDmaBytes != 0) return GH_ERR_UNSUPPORTED;
+    auto lds_of = [&](int c, int d) {
+        size_t l = (size_t)(R + d + 1) * (c + 2 * R) * n2 * vs;
+        if (vt) l += (size_t)(d + 1) * c * n2 * sizeof(S);
+        if (src) l += (size_t)(d + 1) * c * n2 * vs;
+        return l;
+    };
+    // Largest tile (rows) that fits for each prefetch depth; the automatic
+    // choice prefers depth 3 unless that costs more than a quarter of the
+    // rows of depth 2.  Explicit rows/depth (autotuner) override.
+    auto max_rows = [&](int d) {
         int best = 0;
         for (int c = 1; c <= n1; c++) {
-            size_t l = (size_t)P * (c + 2 * R) * n2 * vs;
             int thr = (c * n2 + PPT - 1) / PPT;
             int ncomp = (thr + 63) / 64;
-            if (l > kLdsBytes - 1024 || (ncomp + d) * 64 > kRingMaxThreads) break;
-            if (n1 % c == 0 || c == n1 || best == 0) best = c;
+            if (lds_of(c, d) > kLdsBytes - 1024 || (ncomp + d) * 64 > kRingMaxThreads) break;
+            best = c;
         }
-        if (h->ty_req > 0 && h->ty_req <= best) best = h->ty_req;
-        if (best > ty) {
-            ty = best;
-            D = d;
-            lds = (size_t)P * (best + 2 * R) * n2 * vs;
-        }
+        return best;
+    };
+    auto divisor_rows = [&](int c) {   // prefer a divisor of n1 >= c / 2
+        for (int t = c; t >= std::max(1, c / 2); t--)
+            if (n1 % t == 0) return t;
+        return c;
+    };
+    int ty = 0, D = 0;
+    if (h->depth_req >= 2) {
+        D = h->depth_req;
+        int mr = max_rows(D);
+        ty = h->ty_req > 0 ? std::min(h->ty_req, mr) : divisor_rows(mr);
+    } else {
+        int r3 = divisor_rows(max_rows(3)), r2 = divisor_rows(max_rows(2));
+        if (4 * r3 >= 3 * r2) { D = 3; ty = r3; } else { D = 2; ty = r2; }
+        if (h->ty_req > 0) ty = std::min(h->ty_req, max_rows(D));
     }
-    if (ty == 0) return GH_ERR_UNSUPPORTED;
+    if (ty <= 0) return GH_ERR_UNSUPPORTED;
+    const size_t lds = lds_of(ty, D);
+    RingLayout lay;
+    lay.psi_slot = (ty + 2 * R) * n2;
+    lay.aux_slot = ty * n2;
+    lay.v_off = (R + D + 1) * lay.psi_slot * vs;
+    lay.s_off = lay.v_off + (vt ? (D + 1) * lay.aux_slot * (int)sizeof(S) : 0);
     RingGeom rg;
     CfdGeom& g = rg.g;
     g.n0 = n0; g.n1 = n1; g.n2 = n2;
@@ -730,6 +802,15 @@ int launch_ring_r(const gh_stencil* h, const void* in, void* out,
     g.nxc = (n0 + g.lx - 1) / g.lx;
     rg.nload = D;
     rg.ncomp = ((ty * n2 + PPT - 1) / PPT + 63) / 64;
+    rg.nbands = (int)nbands;
+    {
+        // one block per CU (LDS-bound): ~32 blocks run together per XCD
+        const int tiles = g.nyt * g.nxc;
+        int bgr = h->bgroup_req > 0 ? h->bgroup_req : std::max(1, 32 / tiles);
+        rg.bgroup = std::min<int>(bgr, (int)nbands);
+        const char* nt = getenv("GPAW_AMD_NTSTORE");
+        rg.nt_store = nt ? atoi(nt) : 1;
+    }
     CfdCoefs<S> cf;
     cf.c0 = (S)(alpha * h->c0 + gamma);
     for (int c = 0; c < 3; c++)
@@ -749,7 +830,7 @@ int launch_ring_r(const gh_stencil* h, const void* in, void* out,
         GH_CHECK_HIP(hipFuncSetAttribute((const void*)kfn,                     \
             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));            \
         hipLaunchKernelGGL(kfn, grid, block, lds, st, (const V*)in, (V*)out,   \
-                           vtp, srcp, (S)beta, cf, rg, ph);                    \
+                           vtp, srcp, (S)beta, cf, rg, ph, lay);               \
     } while (0)
     if (vt && src) GH_LAUNCH_RING(true, true);
     else if (vt) GH_LAUNCH_RING(true, false);
@@ -856,6 +937,8 @@ int gh_stencil_create(gh_stencil** out, int ncoefs, const double* coefs,
     {
         const char* e = getenv("GPAW_AMD_STENCIL");
         h->reg_only = e && std::string(e) == "reg";
+        const char* bg = getenv("GPAW_AMD_BGROUP");
+        h->bgroup_req = bg ? atoi(bg) : 0;
     }
     h->offs.assign(offsets_pc, offsets_pc + 3 * ncoefs);
     h->R = 0;
@@ -902,10 +985,13 @@ int gh_stencil_destroy(gh_stencil* h) {
 
 double gh_stencil_diagonal(const gh_stencil* h) { return h ? h->diag : 0.0; }
 
-int gh_stencil_set_tile(gh_stencil* h, int rows, int planes) {
-    GH_REQUIRE(h && rows >= 0 && planes >= 0, GH_ERR_ARG, "set_tile: bad args");
+int gh_stencil_set_tile(gh_stencil* h, int rows, int planes, int depth) {
+    GH_REQUIRE(h && rows >= 0 && planes >= 0 && depth >= 0 && depth <= 3 &&
+                   depth != 1,
+               GH_ERR_ARG, "set_tile: bad args");
     h->ty_req = rows;
     h->lx_req = planes;
+    h->depth_req = depth;
     return GH_OK;
 }
 

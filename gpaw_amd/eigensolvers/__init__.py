@@ -1,0 +1,2 @@
+"""Eigensolver drivers over device tensors (FD mode)."""
+from .rmmdiis import RMMDIIS, FDWaveFunctions  # noqa: F401

@@ -1,0 +1,241 @@
+"""Blocked RMM-DIIS on the GPU (gpaw/eigensolvers/rmmdiis.py counterpart).
+
+FDWaveFunctions holds one k-point's wave functions psit (nbands, n0, n1, n2)
+on the device with eigenvalues eps and projections P = <p|psit>.
+RMMDIIS.iterate restates Eigensolver.iterate + RMMDIIS.iterate_one_k_point
+(eigensolver.py:121-147, rmmdiis.py:63-254):
+
+  1. subspace diagonalisation (eigensolver.py:175-236): H_nm =
+     dv <psi_n|T+v|psi_m> + sum_a P_n^* dH_a P_m, eigh, rotate psit, Htpsit
+     and P by the eigenvectors;
+  2. residuals R = (T+v) psi - eps psi + sum_a p_a (P dH_a - eps P dS_a)
+     (eigensolver.py:153-172);
+  3. per block of B bands: error += w_n int |R_n|^2; dpsi = M R
+     (preconditioner); dR = (H - eps S) dpsi; lambda = -<R|dR>/<dR|dR>;
+     psi += lambda dpsi, R += lambda dR; niter - 1 DIIS steps (minimise
+     |sum_i alpha_i R_i| with sum_i alpha_i = 1 over the stored iterates);
+     final trial step psi += 0.1 M R;
+  4. orthonormalisation (fdpw.py:394-424): S = dv psi psi^H + P dS P^H =
+     L L^H, psi <- L^-1 psi, P <- L^-1 P.
+
+Grid work (stencils, projectors, V-cycle, band-wise dots and axpys) runs in
+libgpawhip; the nbands x nbands algebra (S, H, eigh, Cholesky, rotations)
+runs on the GPU in FP64 through torch (rocBLAS / rocSOLVER).
+"""
+import numpy as np
+import torch
+
+from .. import blas
+
+
+class FDWaveFunctions:
+    """One k-point's wave functions on the device (fdpw.py counterpart)."""
+
+    def __init__(self, gd, psit, pt, preconditioner, phase_cd=None):
+        self.gd = gd
+        self.psit = psit
+        self.pt = pt
+        self.preconditioner = preconditioner
+        self.phase_cd = phase_cd
+        self.dtype = psit.dtype
+        self.nbands = psit.shape[0]
+        self.dv = gd.dv
+        self.eps = torch.zeros(self.nbands, dtype=torch.float64,
+                               device='cuda')
+        self.P = None
+        self.orthonormalized = False
+
+    @staticmethod
+    def flat(a):
+        return a.reshape(a.shape[0], -1)
+
+    def projections(self):
+        if self.P is None and self.pt is not None:
+            self.P = self.pt.integrate_M(self.psit)
+        return self.P
+
+    def overlap(self, ham):
+        """S' = dv psi psi^H + P dS P^H (the complex conjugate of
+        S_nm = <psi_n|S|psi_m>; equal to it for real wave functions)."""
+        psi = self.flat(self.psit)
+        S = self.dv * (psi @ psi.conj().T)
+        P = self.projections()
+        if P is not None and ham is not None and ham.dS_flat is not None:
+            S = S + ham.dS(P) @ P.conj().T
+        return S
+
+    def orthonormalize(self, ham=None):
+        """psi <- L^-1 psi, P <- L^-1 P with L L^H = S' (fdpw.py:394-424)."""
+        S = self.overlap(ham)
+        L = torch.linalg.cholesky(S)
+        Linv = torch.linalg.solve_triangular(
+            L, torch.eye(len(S), dtype=S.dtype, device=S.device), upper=False)
+        psi = self.flat(self.psit)
+        self.psit.copy_((Linv @ psi).reshape(self.psit.shape))
+        if self.P is not None:
+            self.P = Linv @ self.P
+        self.orthonormalized = True
+
+
+class _Kpt:
+    def __init__(self, phase_cd):
+        self.phase_cd = phase_cd
+
+
+class RMMDIIS:
+    """RMM-DIIS eigensolver (rmmdiis.py:10-50 parameters)."""
+
+    def __init__(self, keep_htpsit=True, blocksize=10, niter=3, rtol=1e-16,
+                 limit_lambda=False, trial_step=0.1):
+        if not keep_htpsit:
+            raise NotImplementedError('keep_htpsit=False')
+        self.keep_htpsit = keep_htpsit
+        self.blocksize = blocksize
+        self.niter = niter
+        self.rtol = rtol
+        self.limit_lambda = limit_lambda
+        self.trial_step = trial_step
+        self.orthonormalization_required = True
+        self.error = None
+
+    # ----------------------------------------------------------- pieces --
+    def apply_h_local(self, ham, wfs, a, out):
+        ham.apply_local(a, out, wfs.phase_cd)
+
+    def calculate_residuals(self, ham, wfs, psit, P, eps, R):
+        """R <- R - eps psit + sum_a p_a (P dH_a - eps P dS_a), where R holds
+        (T + v) psit on entry (eigensolver.py:153-172)."""
+        blas.multi_axpy(-eps, psit, R)
+        if ham.pt is not None:
+            ham.pt.add_M(R, ham.dH(P, eps_x=eps))
+
+    def subspace_diagonalize(self, ham, wfs):
+        """Rotate psit, P to the eigenvectors of the subspace Hamiltonian;
+        returns (T + v) psit of the rotated wave functions."""
+        psit = wfs.psit
+        Htpsit = torch.empty_like(psit)
+        self.apply_h_local(ham, wfs, psit, Htpsit)
+        psi = wfs.flat(psit)
+        hpsi = wfs.flat(Htpsit)
+        H = wfs.dv * (psi.conj() @ hpsi.T)
+        P = wfs.projections()
+        if P is not None:
+            H = H + P.conj() @ ham.dH(P).T
+        H = 0.5 * (H + H.conj().T)
+        eps, C = torch.linalg.eigh(H)
+        U = C.T.contiguous()
+        psit.copy_((U @ psi).reshape(psit.shape))
+        Htpsit.copy_((U @ hpsi).reshape(psit.shape))
+        if P is not None:
+            wfs.P = U @ P
+        wfs.eps = eps.to(torch.float64).contiguous()
+        return Htpsit
+
+    def integrate(self, wfs, a, b):
+        """Re <a_x|b_x> per band."""
+        return blas.multi_dotc(a, b, scale=wfs.dv)
+
+    def precondition(self, wfs, R, kpt):
+        return wfs.preconditioner(R, kpt)
+
+    def residual_of(self, ham, wfs, x, eps, out):
+        """out = (H - eps S) x."""
+        self.apply_h_local(ham, wfs, x, out)
+        P = ham.pt.integrate_M(x) if ham.pt is not None else None
+        self.calculate_residuals(ham, wfs, x, P, eps, out)
+        return out
+
+    # ------------------------------------------------------------ driver --
+    def iterate(self, ham, wfs, weights=None):
+        """One sweep over all bands (Eigensolver.iterate for one k-point).
+        Returns the error sum_n w_n int |R_n|^2 dv (weights w_n default 1)."""
+        if not wfs.orthonormalized:
+            wfs.orthonormalize(ham)
+        error = self.iterate_one_k_point(ham, wfs, weights)
+        if self.orthonormalization_required:
+            wfs.P = None
+            wfs.orthonormalize(ham)
+            wfs.projections()
+        self.error = error
+        return error
+
+    def iterate_one_k_point(self, ham, wfs, weights=None):
+        nb = wfs.nbands
+        if weights is None:
+            weights = torch.ones(nb, dtype=torch.float64, device='cuda')
+        else:
+            weights = torch.as_tensor(weights, dtype=torch.float64,
+                                      device='cuda')
+        R = self.subspace_diagonalize(ham, wfs)
+        psit = wfs.psit
+        self.calculate_residuals(ham, wfs, psit, wfs.P, wfs.eps, R)
+        kpt = _Kpt(wfs.phase_cd)
+        B = self.blocksize
+        errors = []
+        dR_full = None
+        for n1 in range(0, nb, B):
+            n2 = min(n1 + B, nb)
+            psib = psit[n1:n2]
+            Rb = R[n1:n2]
+            eps = wfs.eps[n1:n2].contiguous()
+            errors.append(weights[n1:n2] * self.integrate(wfs, Rb, Rb))
+            if self.niter > 1:
+                psi_hist = [psib.clone()]
+                R_hist = [Rb.clone()]
+            dpsit = self.precondition(wfs, Rb, kpt)
+            if dR_full is None:
+                dR_full = torch.empty_like(dpsit)
+            dR = dR_full[:n2 - n1]
+            self.residual_of(ham, wfs, dpsit, eps, dR)
+            RdR = self.integrate(wfs, dR, Rb)
+            dRdR = self.integrate(wfs, dR, dR)
+            lam = -RdR / dRdR
+            if self.limit_lambda:
+                lam = self._limit(lam)
+            blas.multi_axpy(lam, dpsit, psib)
+            blas.multi_axpy(lam, dR, Rb)
+            for nit in range(1, self.niter):
+                psi_hist.append(psib.clone())
+                R_hist.append(Rb.clone())
+                if nit > 1 or self.limit_lambda:
+                    self._diis_combine(wfs, psi_hist, R_hist, psib, Rb, nit)
+                if nit < self.niter - 1:
+                    dpsit = self.precondition(wfs, Rb, kpt)
+                    blas.multi_axpy(lam, dpsit, psib)
+                    self.residual_of(ham, wfs, psib, eps, Rb)
+            dpsit = self.precondition(wfs, Rb, kpt)
+            if self.trial_step is not None:
+                lam = torch.full((n2 - n1,), self.trial_step,
+                                 dtype=torch.float64, device='cuda')
+            blas.multi_axpy(lam, dpsit, psib)
+        return float(torch.cat(errors).sum())
+
+    def _limit(self, lam):
+        lower = self.limit_lambda['lower']
+        upper = self.limit_lambda['upper']
+        if self.limit_lambda.get('absolute', False):
+            lam = torch.where(lam.abs() < lower, lower * lam.sign(), lam)
+            lam = torch.where(lam.abs() > upper, upper * lam.sign(), lam)
+        else:
+            lam = lam.clamp(lower, upper)
+        return lam
+
+    def _diis_combine(self, wfs, psi_hist, R_hist, psib, Rb, nit):
+        """Per band: alpha minimising |sum_i alpha_i R_i| subject to
+        sum_i alpha_i = 1 over the nit + 1 stored iterates, then
+        psi = sum_i alpha_i psi_i, R = sum_i alpha_i R_i
+        (rmmdiis.py:200-239)."""
+        Rh = torch.stack([wfs.flat(r) for r in R_hist[:nit + 1]], 1)
+        Ph = torch.stack([wfs.flat(p) for p in psi_hist[:nit + 1]], 1)
+        nbk = Rh.shape[0]
+        # R_nn[b, i, j] = dv <R_i|R_j>
+        R_nn = wfs.dv * torch.bmm(Rh.conj(), Rh.transpose(1, 2).contiguous())
+        A = -torch.ones((nbk, nit + 2, nit + 2), dtype=R_nn.dtype,
+                        device='cuda')
+        A[:, :nit + 1, :nit + 1] = R_nn
+        A[:, -1, -1] = 0.0
+        x = torch.zeros((nbk, nit + 2, 1), dtype=R_nn.dtype, device='cuda')
+        x[:, -1, 0] = -1.0
+        alpha = torch.linalg.solve(A, x)[:, :-1, 0]
+        psib.copy_(torch.bmm(alpha[:, None, :], Ph).reshape(psib.shape))
+        Rb.copy_(torch.bmm(alpha[:, None, :], Rh).reshape(Rb.shape))

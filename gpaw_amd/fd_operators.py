@@ -112,10 +112,11 @@ class FDOperator:
     def get_diagonal_element(self):
         return self._lib.gh_stencil_diagonal(self.handle)
 
-    def set_tile(self, rows=0, planes=0):
-        """Tile shape of the stencil kernel (0 = automatic)."""
+    def set_tile(self, rows=0, planes=0, depth=0):
+        """Tile shape of the stencil kernel: rows per block, x-planes per
+        block, prefetch depth (0 = automatic)."""
         _lib.check(self._lib.gh_stencil_set_tile(self.handle, int(rows),
-                                                 int(planes)))
+                                                 int(planes), int(depth)))
 
 
 def Laplace(gd, scale=1.0, n=1, dtype=float, xp=None):

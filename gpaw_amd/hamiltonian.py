@@ -14,7 +14,8 @@ from ._arrays import Staged, real_dtype, stream_ptr, torch_dtype
 
 
 class FDHamiltonian:
-    def __init__(self, gd, kin, vt_G, pt=None, dH_aii=None, dtype=float):
+    def __init__(self, gd, kin, vt_G, pt=None, dH_aii=None, dtype=float,
+                 dS_aii=None):
         self.gd = gd
         self.kin = kin
         self.dtype = dtype
@@ -23,31 +24,49 @@ class FDHamiltonian:
                 real_dtype(dtype)).contiguous()
         self.pt = pt
         self._lib = _lib.lib()
+        self.dS_flat = None
         if pt is not None:
             assert dH_aii is not None
-            blocks = []
-            for a in range(len(pt.Mmax_a)):
-                dH = np.asarray(dH_aii[a], float)
-                assert dH.shape == (pt.Mmax_a[a],) * 2
-                blocks.append(dH.ravel())
-            self.dH_flat = torch.as_tensor(np.concatenate(blocks),
-                                           device='cuda')
+            self.dH_flat = self._blocks(dH_aii)
             self._dH_host = dH_aii
+            if dS_aii is not None:
+                self.dS_flat = self._blocks(dS_aii)
 
-    def dH(self, P_xM, out=None, eps_x=None, dS_flat=None):
+    def _blocks(self, X_aii):
+        """Per-atom (nm_a, nm_a) matrices, packed atom after atom."""
+        blocks = []
+        for a in range(len(self.pt.Mmax_a)):
+            X = np.asarray(X_aii[a], float)
+            assert X.shape == (self.pt.Mmax_a[a],) * 2
+            blocks.append(X.ravel())
+        return torch.as_tensor(np.concatenate(blocks), device='cuda')
+
+    def dH(self, P_xM, out=None, eps_x=None):
         """out[x, M_a + i] = sum_j P[x, M_a + j] dH_a[j, i]
-        (- eps_x[x] sum_j P dS_a[j, i])."""
+        (- eps_x[x] sum_j P[x, M_a + j] dS_a[j, i] when eps_x is given and
+        the Hamiltonian has overlap corrections dS)."""
+        return self._apply_blocks(self.dH_flat, P_xM, out, eps_x,
+                                  self.dS_flat)
+
+    def dS(self, P_xM, out=None):
+        """out[x, M_a + i] = sum_j P[x, M_a + j] dS_a[j, i] (None: dS = 0)."""
+        if self.dS_flat is None:
+            return None
+        return self._apply_blocks(self.dS_flat, P_xM, out, None, None)
+
+    def _apply_blocks(self, X_flat, P_xM, out, eps_x, dS_flat):
+        P_xM = P_xM.contiguous()
         if out is None:
             out = torch.empty_like(P_xM)
         nb = int(np.prod(P_xM.shape[:-1], dtype=np.int64))
         eps = None
-        if eps_x is not None:
+        if eps_x is not None and dS_flat is not None:
             eps = torch.as_tensor(eps_x, dtype=torch.float64,
                                   device='cuda').contiguous()
         _lib.check(self._lib.gh_lfc_apply_dH(
-            self.pt.handle, P_xM.data_ptr(), self.dH_flat.data_ptr(),
+            self.pt.handle, P_xM.data_ptr(), X_flat.data_ptr(),
             eps.data_ptr() if eps is not None else None,
-            dS_flat.data_ptr() if dS_flat is not None else None,
+            dS_flat.data_ptr() if eps is not None else None,
             out.data_ptr(), nb, stream_ptr()))
         return out
 

@@ -79,9 +79,10 @@ int gh_stencil_apply(const gh_stencil* h, const void* in_dev, void* out_dev,
                      const void* vt_dev, const void* src_dev, double beta,
                      void* stream);
 
-/* Tile choice of the stencil kernel (rows of the (y, z) tile per block and
- * planes per block along x); 0 = automatic.  Used by the autotuner. */
-int gh_stencil_set_tile(gh_stencil* h, int rows, int planes);
+/* Tile choice of the stencil kernel: rows of the (y, z) tile per block,
+ * planes per block along x, and the prefetch depth in planes (2 or 3) of
+ * the LDS-ring kernel; 0 = automatic.  Used by the autotuner. */
+int gh_stencil_set_tile(gh_stencil* h, int rows, int planes, int depth);
 
 /* -------------------------------------------------------- transformers --
  * Replaces cgpaw.Transformer (reference c/transformers.c:202-252, created

@@ -1,0 +1,130 @@
+"""GPU RMM-DIIS driver, subspace diagonalisation and orthonormalisation
+against the oracle (oracle/rmmdiis.py) and the dense generalized eigenvalue
+problem.  Tolerances: 1e-12 for orthonormalisation and subspace
+eigenvalues, 1e-10 (relative) for one full sweep -- the sweep goes through
+eigh, a V-cycle and DIIS solves, so rounding differences of 1e-15 grow by
+the conditioning of those steps -- and 1e-10 for converged eigenvalues."""
+import numpy as np
+import pytest
+import scipy.linalg as sl
+
+torch = pytest.importorskip('torch')
+
+pytestmark = pytest.mark.gpu
+
+if not torch.cuda.is_available():
+    pytest.skip('no GPU', allow_module_level=True)
+
+from gpaw_amd.eigensolvers import RMMDIIS, FDWaveFunctions  # noqa: E402
+from gpaw_amd.fd_operators import Laplace  # noqa: E402
+from gpaw_amd.grid_descriptor import GridDescriptor  # noqa: E402
+from gpaw_amd.hamiltonian import FDHamiltonian  # noqa: E402
+from gpaw_amd.lfc import LFC  # noqa: E402
+from gpaw_amd.preconditioner import Preconditioner  # noqa: E402
+from gpaw_amd.spline import Spline  # noqa: E402
+
+from oracle import rmmdiis as orm  # noqa: E402
+from test_oracle_rmmdiis import dense_pair, small_system, start_guess  # noqa
+
+CASES = [None, (0.25, -0.1, 0.3)]
+
+
+def gpu_system(ham_o):
+    raw = ham_o.raw
+    N, h, k_c = raw['N'], raw['h'], raw['k_c']
+    dtype = float if k_c is None else complex
+    gd = GridDescriptor((N,) * 3, (N * h,) * 3)
+    spl = [[Spline.from_data(l, raw['rcut'], f) for l, f in fs]
+           for fs in raw['splines']]
+    kd = None if k_c is None else np.array([k_c])
+    pt = LFC(gd, spl, kd, dtype=dtype)
+    pt.set_positions(raw['spos'])
+    kin = Laplace(gd, -0.5, 3, dtype)
+    ham = FDHamiltonian(gd, kin, raw['vt'], pt, raw['dH'], dtype=dtype,
+                        dS_aii=raw['dS'])
+    pc = Preconditioner(gd, kin, dtype)
+    return gd, ham, pc
+
+
+def dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def host(t):
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+def align(a, b):
+    """Per-band phase of a that best matches b (eigenvectors are defined up
+    to a phase)."""
+    A, B = orm.flat(a), orm.flat(b)
+    ov = np.sum(A.conj() * B, axis=1)
+    return a * (ov / abs(ov)).reshape((-1,) + (1,) * (a.ndim - 1))
+
+
+@pytest.mark.parametrize('k_c', CASES)
+def test_orthonormalize_vs_oracle(k_c):
+    ham_o, _, rng = small_system(N=12, h=0.4, k_c=k_c, seed=1)
+    gd, ham, pc = gpu_system(ham_o)
+    dtype = float if k_c is None else complex
+    psi = rng.standard_normal((5, 12, 12, 12)).astype(dtype)
+    wfs = FDWaveFunctions(gd, dev(psi), ham.pt, pc, ham_o.phase_cd)
+    wfs.orthonormalize(ham)
+    ref, Pref = orm.orthonormalize(ham_o, psi)
+    out = host(wfs.psit)
+    assert abs(out - ref).max() < 1e-12 * abs(ref).max()
+    assert abs(host(wfs.P) - Pref).max() < 1e-12 * abs(Pref).max()
+
+
+@pytest.mark.parametrize('k_c', CASES)
+def test_subspace_diagonalize_vs_oracle(k_c):
+    ham_o, _, rng = small_system(N=12, h=0.4, k_c=k_c, seed=2)
+    gd, ham, pc = gpu_system(ham_o)
+    dtype = float if k_c is None else complex
+    psi, _ = orm.orthonormalize(
+        ham_o, rng.standard_normal((6, 12, 12, 12)).astype(dtype))
+    wfs = FDWaveFunctions(gd, dev(psi), ham.pt, pc, ham_o.phase_cd)
+    Ht = RMMDIIS().subspace_diagonalize(ham, wfs)
+    eps, psi2, Ht2, P2 = orm.subspace_diagonalize(ham_o, psi)
+    assert abs(host(wfs.eps) - eps).max() < 1e-12 * abs(eps).max()
+    out = host(wfs.psit)
+    aligned = align(out, psi2)
+    assert abs(aligned - psi2).max() < 1e-11 * abs(psi2).max()
+    assert abs(align(host(Ht), Ht2) - Ht2).max() < 1e-11 * abs(Ht2).max()
+
+
+@pytest.mark.parametrize('k_c', CASES)
+def test_rmmdiis_sweep_vs_oracle(k_c):
+    """One Eigensolver.iterate (3 DIIS iterations, blocks of 4 over 6 bands,
+    so the last block is partial) from the same start."""
+    ham_o, pc_o, rng = small_system(N=12, h=0.4, k_c=k_c, seed=3)
+    gd, ham, pc = gpu_system(ham_o)
+    Hm, Sm = dense_pair(ham_o)
+    nb = 6
+    w, W = sl.eigh(Hm, Sm, subset_by_index=[0, nb - 1])
+    psi, _ = orm.orthonormalize(ham_o, start_guess(ham_o, rng, nb, W, 0.3))
+    err_o, eps_o, psi_o = orm.rmmdiis_sweep(ham_o, psi.copy(), pc_o,
+                                            blocksize=4)
+    wfs = FDWaveFunctions(gd, dev(psi), ham.pt, pc, ham_o.phase_cd)
+    wfs.orthonormalized = True
+    err = RMMDIIS(blocksize=4).iterate(ham, wfs)
+    assert abs(err - err_o) < 1e-10 * err_o
+    assert abs(host(wfs.eps) - eps_o).max() < 1e-12 * abs(eps_o).max()
+    out = host(wfs.psit)
+    assert abs(align(out, psi_o) - psi_o).max() < 1e-10 * abs(psi_o).max()
+
+
+@pytest.mark.parametrize('k_c', CASES)
+def test_rmmdiis_converges_to_dense(k_c):
+    ham_o, _, rng = small_system(N=12, h=0.4, k_c=k_c, seed=4)
+    gd, ham, pc = gpu_system(ham_o)
+    Hm, Sm = dense_pair(ham_o)
+    nb = 6
+    w, W = sl.eigh(Hm, Sm, subset_by_index=[0, nb - 1])
+    psi = start_guess(ham_o, rng, nb, W, 0.3)
+    wfs = FDWaveFunctions(gd, dev(psi), ham.pt, pc, ham_o.phase_cd)
+    es = RMMDIIS(blocksize=4)
+    errors = [es.iterate(ham, wfs) for _ in range(30)]
+    assert errors[-1] < 1e-6 * errors[0]
+    assert abs(host(wfs.eps)[:4] - w[:4]).max() < 1e-10

@@ -280,6 +280,28 @@ def test_lfc_vs_oracle(dtype, k_c):
         assert relerr(host(c_axi[at]), ref[:, M:M + 8]) < 1e-13
 
 
+@pytest.mark.parametrize('dtype,k_c', [(np.float64, None),
+                                       (np.complex128, (0.25, -0.3, 0.1))])
+def test_lfc_self_overlap(dtype, k_c):
+    """Cell (2.5 x 2.5 x 3 Bohr) shorter than the sphere diameter (4 Bohr):
+    every sphere overlaps its own periodic images, so one atom has several
+    entries on one grid point (the add must not race within the atom)."""
+    rng = np.random.default_rng(12)
+    N_c = (10, 10, 12)
+    gd, lfc, olf = _lfc_pair(rng, N_c, 0.25, 3, dtype, k_c)
+    # more entries than 3 atoms x grid points: some atom repeats a point
+    assert lfc.nentries > 3 * np.prod(N_c)
+    a = rand(rng, (6,) + N_c, dtype)
+    c = lfc.integrate_M(dev(a))
+    ref = olf.integrate(a)
+    assert relerr(host(c), ref) < 1e-13
+    coef = rand(rng, ref.shape, dtype)
+    b = dev(a.copy())
+    lfc.add_M(b, dev(coef))
+    refb = olf.add(a.copy(), coef)
+    assert relerr(host(b), refb) < 1e-13
+
+
 def test_lfc_deterministic():
     """test_lf.py:8-25 -- add twice gives identical bits."""
     rng = np.random.default_rng(9)
