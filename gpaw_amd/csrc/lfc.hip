@@ -34,8 +34,9 @@ namespace gh {
 
 constexpr int kLfcMaxNm = 64;   // functions per atom supported
 constexpr int kIntChunk = 256;  // entries staged per integrate step
-constexpr int kAddBT = 4;       // bands per add wavefront
-constexpr int kAddChunk = 512;  // entries per add wavefront
+constexpr int kAddBT = 32;      // bands per add wavefront
+constexpr int kAddGrp = 8;      // bands loaded together in add
+constexpr int kAddChunk = 64;   // entries per add wavefront (one per lane)
 
 template <typename V> struct IntBT { static constexpr int value = 16; };
 template <> struct IntBT<cplx<double>> { static constexpr int value = 8; };
@@ -130,12 +131,14 @@ lfc_integrate_kernel(const V* __restrict__ a, V* __restrict__ c,
     }
 }
 
-// add: one wavefront per (unit of the colour, entry chunk, band tile).  A
-// unit is a contiguous entry range of one atom in which every grid point
-// occurs once (an atom whose sphere overlaps its own periodic image has
-// several units).  lane = entry; the tile's coefficients c[x, M_a + m] are
-// wave-uniform (scalar registers when the function count is known at
-// compile time).
+// add: one wavefront per (unit of the colour, chunk of 64 entries, band
+// tile of kAddBT bands).  A unit is a contiguous entry range of one atom in
+// which every grid point occurs once (an atom whose sphere overlaps its own
+// periodic image has several units).  lane = entry: the lane keeps its
+// entry's grid index, phase and function values (registers when the count
+// is known at compile time) and sweeps the tile's bands, kAddGrp at a time
+// (all loads of a group issued before its stores); the coefficients
+// c[x, M_a + m] are wave-uniform (scalar loads).
 template <typename V, typename S, int NM>
 __global__ void __launch_bounds__(64)
 lfc_add_kernel(const V* __restrict__ c, V* __restrict__ a,
@@ -161,40 +164,39 @@ lfc_add_kernel(const V* __restrict__ c, V* __restrict__ a,
     const int M0 = M_ptr[atom];
     const int nm = NM > 0 ? NM : M_ptr[atom + 1] - M0;
     const int nb = min(kAddBT, nbands - x0);
-    const S* __restrict__ Aa = A + A_off[atom];
-    for (int64_t e = eb + threadIdx.x; e < e1; e += 64) {
-        const int64_t G = G_e[e];
-        const S* __restrict__ Ae = Aa + (e - e0) * nm;
-        V s[kAddBT];
+    const int64_t e = eb + threadIdx.x;
+    if (e >= e1) return;
+    const int64_t G = G_e[e];
+    const S* __restrict__ Ae = A + A_off[atom] + (e - e0) * nm;
+    S Ar[NM > 0 ? NM : 1];
+    if constexpr (NM > 0) {
 #pragma unroll
-        for (int b = 0; b < kAddBT; b++) s[b] = vzero<V>();
+        for (int m = 0; m < NM; m++) Ar[m] = Ae[m];
+    }
+    S pre = 1, pim = 0;
+    if (is_complex<V>::value && phase_w) {
+        int w = w_e[e];
+        if (w >= 0) { pre = phase_w[2 * w]; pim = -phase_w[2 * w + 1]; }
+    }
+    V* __restrict__ ap = a + (size_t)x0 * ngpts + G;
+    const V* __restrict__ cp = c + (size_t)x0 * Mmax + M0;
+    for (int b0 = 0; b0 < nb; b0 += kAddGrp) {
+        V old[kAddGrp];
 #pragma unroll
-        for (int m = 0; m < (NM > 0 ? NM : 1); m++) {
-            if (NM > 0) {
-                const S Am = Ae[m];
+        for (int g = 0; g < kAddGrp; g++)
+            if (b0 + g < nb) old[g] = ap[(size_t)(b0 + g) * ngpts];
 #pragma unroll
-                for (int b = 0; b < kAddBT; b++)
-                    if (b < nb) s[b] = fma_rs(Am, c[(size_t)(x0 + b) * Mmax + M0 + m], s[b]);
-            }
-        }
-        if (NM == 0) {
-            for (int m = 0; m < nm; m++) {
-                const S Am = Ae[m];
+        for (int g = 0; g < kAddGrp; g++) {
+            if (b0 + g < nb) {
+                const V* __restrict__ cb = cp + (size_t)(b0 + g) * Mmax;
+                V s = vzero<V>();
+                if constexpr (NM > 0) {
 #pragma unroll
-                for (int b = 0; b < kAddBT; b++)
-                    if (b < nb) s[b] = fma_rs(Am, c[(size_t)(x0 + b) * Mmax + M0 + m], s[b]);
-            }
-        }
-        S pre = 1, pim = 0;
-        if (is_complex<V>::value && phase_w) {
-            int w = w_e[e];
-            if (w >= 0) { pre = phase_w[2 * w]; pim = -phase_w[2 * w + 1]; }
-        }
-#pragma unroll
-        for (int b = 0; b < kAddBT; b++) {
-            if (b < nb) {
-                V* p = a + (size_t)(x0 + b) * ngpts + G;
-                *p = vadd(*p, mul_phase(s[b], pre, pim));
+                    for (int m = 0; m < NM; m++) s = fma_rs(Ar[m], cb[m], s);
+                } else {
+                    for (int m = 0; m < nm; m++) s = fma_rs(Ae[m], cb[m], s);
+                }
+                ap[(size_t)(b0 + g) * ngpts] = vadd(old[g], mul_phase(s, pre, pim));
             }
         }
     }

@@ -87,3 +87,54 @@ def full_hpsi_geometry(N, natoms):
     w.proj = projector_data(natoms)
     w.dH = dH_matrices(natoms)
     return w
+
+
+def initial_guess(gd, pos_av, nbands, sigma=1.2, seed=1):
+    """Atom-centred start for RMM-DIIS (an LCAO-like guess): per atom an s
+    and three p Gaussians, exp(-r^2/sigma^2) and (x, y, z) exp(-r^2/sigma^2)
+    (minimum image), atom after atom; bands beyond 4 * natoms are the
+    lowest plane waves cos(G.r + phi) with seeded random phi.  Returns a float64 tensor
+    (nbands, n0, n1, n2) on the GPU (not orthonormalized)."""
+    import torch
+    n_c = tuple(int(n) for n in gd.n_c)
+    L_c = torch.tensor(np.diag(gd.cell_cv), dtype=torch.float64,
+                       device='cuda')
+    axes = [torch.arange(n, dtype=torch.float64, device='cuda') * h
+            for n, h in zip(n_c, np.diag(gd.h_cv))]
+    psi = torch.empty((nbands,) + n_c, dtype=torch.float64, device='cuda')
+    b = 0
+    for p in pos_av:
+        if b >= nbands:
+            break
+        d = []
+        for c in range(3):
+            x = axes[c] - float(p[c])
+            d.append(x - L_c[c] * torch.round(x / L_c[c]))
+        x = d[0][:, None, None]
+        y = d[1][None, :, None]
+        z = d[2][None, None, :]
+        g = torch.exp(-(x * x + y * y + z * z) / sigma**2)
+        for f in (g, x * g, y * g, z * g):
+            if b < nbands:
+                psi[b] = f
+                b += 1
+    if b < nbands:
+        rng = np.random.default_rng(seed)
+        kmax = 1
+        while (2 * kmax + 1)**3 < 2 * (nbands - b) + 1:
+            kmax += 1
+        ks = [(i, j, k) for i in range(-kmax, kmax + 1)
+              for j in range(-kmax, kmax + 1) for k in range(-kmax, kmax + 1)
+              if (i, j, k) > (0, 0, 0)]
+        ks.sort(key=lambda t: (t[0]**2 + t[1]**2 + t[2]**2, t))
+        for i, j, k in ks:
+            if b >= nbands:
+                break
+            phase = (2 * np.pi) * (
+                i * axes[0][:, None, None] / L_c[0] +
+                j * axes[1][None, :, None] / L_c[1] +
+                k * axes[2][None, None, :] / L_c[2])
+            shift = float(rng.uniform(0, 2 * np.pi))
+            psi[b] = torch.cos(phase + shift)
+            b += 1
+    return psi
