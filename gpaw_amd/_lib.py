@@ -32,6 +32,7 @@ _SIGNATURES = {
     'gh_stencil_diagonal': (D, [P]),
     'gh_stencil_apply': (_c.c_int, [P, P, P, I64, PD, P, P, D, P]),
     'gh_stencil_set_tile': (_c.c_int, [P, _c.c_int, _c.c_int, _c.c_int]),
+    'gh_stencil_relax': (_c.c_int, [P, _c.c_int, P, P, P, _c.c_int, D, P]),
     'gh_transformer_create': (_c.c_int, [_c.POINTER(P), PI64, PI64, _c.c_int,
                                          PI64, PI64, PI32, _c.c_int,
                                          _c.c_int]),
@@ -49,6 +50,9 @@ _SIGNATURES = {
                                     P]),
     'gh_multi_axpy': (_c.c_int, [_c.c_int, I64, I64, P, P, P, P]),
     'gh_multi_dotc': (_c.c_int, [_c.c_int, I64, I64, P, P, D, P, P, P]),
+    'gh_multi_gram_work': (I64, [_c.c_int, I64]),
+    'gh_multi_gram': (_c.c_int, [_c.c_int, I64, I64, _c.c_int,
+                                 _c.POINTER(P), D, P, P, P]),
     'gh_spline_coefs': (_c.c_int, [_c.c_int, D, PD, PD]),
     'gh_spline_to_grid': (I64, [_c.c_int, _c.c_int, D, PD, PI64, PI64, PD,
                                 PD, PI64, PI64, PD, PI32]),

@@ -3,6 +3,8 @@
 multi_axpy(a_x, x_xG, y_xG): y[x] += a[x] * x[x]       (mblas.py:13-32)
 multi_dotc(a_xG, b_xG, scale): Re sum conj(a[x]) b[x] * scale, FP64 result
 with FP64 accumulation and a fixed reduction order.
+multi_gram(vecs, scale): G[x, i, j] = scale * sum conj(v_i[x]) v_j[x] for a
+list of k <= 6 arrays of equal shape (one pass over the data).
 """
 import numpy as np
 import torch
@@ -40,3 +42,28 @@ def multi_dotc(a_xG, b_xG, scale=1.0, out=None):
         dtype_code(a_xG.dtype), nb, n, a_xG.data_ptr(), b_xG.data_ptr(),
         float(scale), out.data_ptr(), work.data_ptr(), stream_ptr()))
     return out
+
+
+def multi_gram(vecs, scale=1.0):
+    """Per-band Gram matrices (nb, k, k): float64 for real inputs, complex128
+    for complex ones."""
+    k = len(vecs)
+    v0 = vecs[0]
+    for v in vecs:
+        assert v.shape == v0.shape and v.dtype == v0.dtype
+        assert v.is_contiguous() and v.is_cuda
+    nb, n = _nb_n(v0)
+    L = _lib.lib()
+    cplx = v0.is_complex()
+    out = torch.empty((nb, k, k, 2) if cplx else (nb, k, k),
+                      dtype=torch.float64, device='cuda')
+    nwork = L.gh_multi_gram_work(k, nb)
+    if nwork < 0:
+        raise ValueError('multi_gram: k = %d out of range' % k)
+    work = torch.empty(max(int(nwork), 1), dtype=torch.float64,
+                       device='cuda')
+    ptrs = (_lib.P * k)(*[v.data_ptr() for v in vecs])
+    _lib.check(L.gh_multi_gram(
+        dtype_code(v0.dtype), nb, n, k, ptrs, float(scale), out.data_ptr(),
+        work.data_ptr(), stream_ptr()))
+    return torch.view_as_complex(out) if cplx else out

@@ -88,6 +88,122 @@ __global__ void dotc_final_kernel(const double* __restrict__ partial,
     if (threadIdx.x == 0) out[band] = s * scale;
 }
 
+// Gram matrices of k vectors per band: G[x][i][j] = scale <v_i[x]|v_j[x]>,
+// all k (k + 1) / 2 products in one pass over the data; FP64 accumulation,
+// fixed reduction order.  Complex results are (re, im) pairs.
+constexpr int kGramMax = 6;
+
+template <typename V> struct GramPtrs { const V* v[kGramMax]; };
+
+__device__ __forceinline__ void gram_term(double a, double b, double& re, double&) {
+    re += a * b;
+}
+__device__ __forceinline__ void gram_term(float a, float b, double& re, double&) {
+    re += (double)a * (double)b;
+}
+template <typename S>
+__device__ __forceinline__ void gram_term(cplx<S> a, cplx<S> b, double& re, double& im) {
+    re += (double)a.re * (double)b.re + (double)a.im * (double)b.im;
+    im += (double)a.re * (double)b.im - (double)a.im * (double)b.re;
+}
+
+template <typename V, int K>
+__global__ void __launch_bounds__(256)
+gram_partial_kernel(GramPtrs<V> vp, int64_t n, double* __restrict__ partial) {
+    constexpr int NP = K * (K + 1) / 2;
+    __shared__ double red[4][2 * NP];
+    const int band = blockIdx.y;
+    const int chunk = blockIdx.x;
+    const int64_t per = (n + kDotChunks - 1) / kDotChunks;
+    const int64_t i0 = chunk * per;
+    const int64_t i1 = min(n, i0 + per);
+    double re[NP], im[NP];
+#pragma unroll
+    for (int p = 0; p < NP; p++) re[p] = im[p] = 0.0;
+    for (int64_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+        V x[K];
+#pragma unroll
+        for (int a = 0; a < K; a++) x[a] = vp.v[a][(size_t)band * n + i];
+        int p = 0;
+#pragma unroll
+        for (int a = 0; a < K; a++)
+#pragma unroll
+            for (int b = a; b < K; b++, p++) gram_term(x[a], x[b], re[p], im[p]);
+    }
+#pragma unroll
+    for (int p = 0; p < NP; p++) {
+        double r = re[p], m = im[p];
+        for (int off = 32; off > 0; off >>= 1) {
+            r += __shfl_xor(r, off);
+            m += __shfl_xor(m, off);
+        }
+        if ((threadIdx.x & 63) == 0) {
+            red[threadIdx.x >> 6][2 * p] = r;
+            red[threadIdx.x >> 6][2 * p + 1] = m;
+        }
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < 2 * NP; q += blockDim.x)
+        partial[((size_t)band * 2 * NP + q) * kDotChunks + chunk] =
+            (red[0][q] + red[1][q]) + (red[2][q] + red[3][q]);
+}
+
+template <int K>
+__global__ void gram_final_kernel(const double* __restrict__ partial, double scale,
+                                  int cplx_out, double* __restrict__ out) {
+    constexpr int NP = K * (K + 1) / 2;
+    const int band = blockIdx.x;
+    for (int q = 0; q < 2 * NP; q++) {
+        double s = partial[((size_t)band * 2 * NP + q) * kDotChunks + threadIdx.x];
+        for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+        if (threadIdx.x == 0) {
+            const int p = q / 2, part = q % 2;
+            int a = 0, rem = p;
+            while (rem >= K - a) { rem -= K - a; a++; }
+            const int b = a + rem;
+            s *= scale;
+            if (cplx_out) {
+                double* o = out + (size_t)band * K * K * 2;
+                o[(a * K + b) * 2 + part] = s;
+                o[(b * K + a) * 2 + part] = part ? -s : s;   // Hermitian
+            } else if (!part) {
+                double* o = out + (size_t)band * K * K;
+                o[a * K + b] = s;
+                o[b * K + a] = s;
+            }
+        }
+    }
+}
+
+template <typename V, int K>
+int launch_gram(int64_t nbands, int64_t n, const void* const* vecs, double scale,
+                int cplx_out, double* out, double* work, hipStream_t st) {
+    GramPtrs<V> vp;
+    for (int a = 0; a < K; a++) vp.v[a] = (const V*)vecs[a];
+    hipLaunchKernelGGL((gram_partial_kernel<V, K>), dim3(kDotChunks, (unsigned)nbands),
+                       dim3(256), 0, st, vp, n, work);
+    GH_CHECK_LAUNCH();
+    hipLaunchKernelGGL((gram_final_kernel<K>), dim3((unsigned)nbands), dim3(kDotChunks), 0,
+                       st, (const double*)work, scale, cplx_out, out);
+    GH_CHECK_LAUNCH();
+    return GH_OK;
+}
+
+template <typename V>
+int gram_k(int k, int64_t nbands, int64_t n, const void* const* vecs, double scale,
+           int cplx_out, double* out, double* work, hipStream_t st) {
+    switch (k) {
+        case 1: return launch_gram<V, 1>(nbands, n, vecs, scale, cplx_out, out, work, st);
+        case 2: return launch_gram<V, 2>(nbands, n, vecs, scale, cplx_out, out, work, st);
+        case 3: return launch_gram<V, 3>(nbands, n, vecs, scale, cplx_out, out, work, st);
+        case 4: return launch_gram<V, 4>(nbands, n, vecs, scale, cplx_out, out, work, st);
+        case 5: return launch_gram<V, 5>(nbands, n, vecs, scale, cplx_out, out, work, st);
+        case 6: return launch_gram<V, 6>(nbands, n, vecs, scale, cplx_out, out, work, st);
+    }
+    set_error("multi_gram: k = %d out of range 1..%d", k, kGramMax);
+    return GH_ERR_ARG;
+}
+
 template <typename V, typename S>
 int launch_axpbyz(double alpha, const void* x, double beta, const void* y,
                   void* z, int64_t total, hipStream_t st) {
@@ -157,6 +273,33 @@ int gh_multi_dotc(int dtype, int64_t nbands, int64_t n, const void* a_dev,
                        (const double*)work_dev, scale, out_dev);
     GH_CHECK_LAUNCH();
     return GH_OK;
+}
+
+int64_t gh_multi_gram_work(int k, int64_t nbands) {
+    if (k < 1 || k > kGramMax || nbands < 0) return -1;
+    return (int64_t)k * (k + 1) * kDotChunks * nbands;
+}
+
+int gh_multi_gram(int dtype, int64_t nbands, int64_t n, int k,
+                  const void* const* vecs_dev, double scale, double* out_dev,
+                  double* work_dev, void* stream) {
+    GH_REQUIRE(vecs_dev && out_dev && work_dev && n >= 0 && nbands >= 0, GH_ERR_ARG,
+               "multi_gram: bad argument");
+    GH_REQUIRE(k >= 1 && k <= kGramMax, GH_ERR_ARG, "multi_gram: k = %d out of range 1..%d",
+               k, kGramMax);
+    GH_REQUIRE(nbands < 65536, GH_ERR_ARG, "multi_gram: too many bands per call");
+    for (int a = 0; a < k; a++)
+        GH_REQUIRE(vecs_dev[a], GH_ERR_ARG, "multi_gram: null vector %d", a);
+    if (nbands == 0) return GH_OK;
+    hipStream_t st = (hipStream_t)stream;
+    switch (dtype) {
+        case GH_F64: return gram_k<double>(k, nbands, n, vecs_dev, scale, 0, out_dev, work_dev, st);
+        case GH_C128: return gram_k<cplx<double>>(k, nbands, n, vecs_dev, scale, 1, out_dev, work_dev, st);
+        case GH_F32: return gram_k<float>(k, nbands, n, vecs_dev, scale, 0, out_dev, work_dev, st);
+        case GH_C64: return gram_k<cplx<float>>(k, nbands, n, vecs_dev, scale, 1, out_dev, work_dev, st);
+    }
+    set_error("multi_gram: bad dtype");
+    return GH_ERR_ARG;
 }
 
 int64_t gh_precond_scratch_bytes(const gh_stencil* kin0, const gh_stencil* kin1,

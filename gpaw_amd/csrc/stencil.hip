@@ -290,6 +290,8 @@ struct RingGeom {
     int bgroup;           // bands per scheduling group
     int nt_store;         // streaming stores of the output
     int nbands;
+    int nlogical;         // logical tiles: nbands * nyt * nxc
+    int rounds;           // logical tiles per block (persistent launch)
 };
 
 // Block -> (band, x chunk, y tile).  Logical ids of one XCD are contiguous
@@ -334,8 +336,15 @@ stencil_ring_kernel(const V* __restrict__ in, V* __restrict__ out,
     const int D = rg.nload;
     const int P = R + D + 1;
     const int n2 = g.n2;
+    const int Lbase = xcd_swizzle(blockIdx.x, gridDim.x);
+    // persistent blocks take logical tiles Lbase, Lbase + gridDim.x, ...;
+    // every wave of the block sees the same sequence
+    for (int rd = 0; rd < rg.rounds; rd++) {
+    const int L = Lbase + rd * (int)gridDim.x;
+    if (L >= rg.nlogical) break;
+    if (rd > 0) ring_barrier();   // last reads of the previous tile done
     int band, xc, yt;
-    ring_block_coords(rg, xcd_swizzle(blockIdx.x, gridDim.x), band, xc, yt);
+    ring_block_coords(rg, L, band, xc, yt);
     const int y0 = yt * g.ty;
     const int tye = min(g.ty, g.n1 - y0);
     const int x0 = xc * g.lx;
@@ -440,7 +449,7 @@ stencil_ring_kernel(const V* __restrict__ in, V* __restrict__ out,
                 issue_aux(xp - R);
             }
         }
-        return;
+        continue;
     }
 
     // ------------------------------------------------------ compute --
@@ -543,6 +552,7 @@ stencil_ring_kernel(const V* __restrict__ in, V* __restrict__ out,
 #pragma unroll
             for (int j = 0; j < 2 * R; j++) q[k][j] = q[k][j + 1];
     }
+    }  // rounds
 }
 
 // ------------------------------------------------------ generic stencil --
@@ -629,6 +639,8 @@ struct gh_stencil {
     int ty_req = 0, lx_req = 0, depth_req = 0;
     int bgroup_req = 0;      // bands per scheduling group (GPAW_AMD_BGROUP)
     bool reg_only = false;   // GPAW_AMD_STENCIL=reg selects the register kernel
+    int persist = 0;         // persistent ring launch: 1 one block per CU,
+                             // n >= 8 a grid of n blocks (GPAW_AMD_PERSIST)
 };
 
 using namespace gh;
@@ -803,10 +815,33 @@ int launch_ring_r(const gh_stencil* h, const void* in, void* out,
     rg.nload = D;
     rg.ncomp = ((ty * n2 + PPT - 1) / PPT + 63) / 64;
     rg.nbands = (int)nbands;
+    const int tiles = g.nyt * g.nxc;
+    const int64_t nlogical = nbands * (int64_t)tiles;
+    GH_REQUIRE(nlogical < (1ll << 31), GH_ERR_SHAPE, "stencil: too many blocks");
+    rg.nlogical = (int)nlogical;
+    rg.rounds = 1;
+    int64_t nblocks = nlogical;
+    int per_xcd = 32;                     // blocks running together per XCD
+    if (h->persist) {
+        // one resident block per CU walks its XCD's tiles band group by band
+        // group, so all blocks of an XCD stay on the same planes
+        int64_t n = h->persist;
+        if (n < kNumXCD) {
+            int ncu = 0, dev = 0;
+            GH_CHECK_HIP(hipGetDevice(&dev));
+            GH_CHECK_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+            const int per_cu = std::max<int>(1, (int)(kLdsBytes / std::max<size_t>(lds, 1)));
+            n = (int64_t)ncu * per_cu;
+        }
+        n -= n % kNumXCD;
+        if (n >= kNumXCD && n < nlogical) {
+            nblocks = n;
+            rg.rounds = (int)((nlogical + n - 1) / n);
+            per_xcd = (int)(n / kNumXCD);
+        }
+    }
     {
-        // one block per CU (LDS-bound): ~32 blocks run together per XCD
-        const int tiles = g.nyt * g.nxc;
-        int bgr = h->bgroup_req > 0 ? h->bgroup_req : std::max(1, 32 / tiles);
+        int bgr = h->bgroup_req > 0 ? h->bgroup_req : std::max(1, per_xcd / tiles);
         rg.bgroup = std::min<int>(bgr, (int)nbands);
         const char* nt = getenv("GPAW_AMD_NTSTORE");
         rg.nt_store = nt ? atoi(nt) : 1;
@@ -819,8 +854,6 @@ int launch_ring_r(const gh_stencil* h, const void* in, void* out,
             cf.cp[c][j] = (S)(alpha * h->cp[c][j]);
         }
     PhaseArgs<S> ph = phase_args<S>(phase_cd);
-    int64_t nblocks = nbands * (int64_t)g.nyt * g.nxc;
-    GH_REQUIRE(nblocks < (1ll << 31), GH_ERR_SHAPE, "stencil: too many blocks");
     dim3 grid((unsigned)nblocks), block((rg.ncomp + rg.nload) * 64);
     const S* vtp = (const S*)vt;
     const V* srcp = (const V*)src;
@@ -939,6 +972,8 @@ int gh_stencil_create(gh_stencil** out, int ncoefs, const double* coefs,
         h->reg_only = e && std::string(e) == "reg";
         const char* bg = getenv("GPAW_AMD_BGROUP");
         h->bgroup_req = bg ? atoi(bg) : 0;
+        const char* pe = getenv("GPAW_AMD_PERSIST");
+        h->persist = pe ? std::max(0, atoi(pe)) : 0;
     }
     h->offs.assign(offsets_pc, offsets_pc + 3 * ncoefs);
     h->R = 0;
@@ -1002,6 +1037,39 @@ int gh_stencil_apply(const gh_stencil* h, const void* in_dev, void* out_dev,
     return gh::stencil_apply_scaled(h, in_dev, out_dev, nbands, phase_cd,
                                     vt_dev, src_dev, beta, 1.0, 0.0,
                                     (hipStream_t)stream);
+}
+
+int gh_stencil_relax(const gh_stencil* h, int relax_method, void* f_dev,
+                     const void* s_dev, void* work_dev, int nrelax, double w,
+                     void* stream) {
+    GH_REQUIRE(h && f_dev && s_dev && work_dev, GH_ERR_ARG, "relax: null argument");
+    GH_REQUIRE(nrelax >= 0, GH_ERR_ARG, "relax: nrelax < 0");
+    GH_REQUIRE(h->dtype == GH_F64 || h->dtype == GH_F32, GH_ERR_ARG,
+               "relax: real grids only");
+    GH_REQUIRE(relax_method == 2, GH_ERR_UNSUPPORTED,
+               "relax: method %d not provided (only Jacobi, method 2; "
+               "Gauss-Seidel, method 1, is a sequential sweep)", relax_method);
+    GH_REQUIRE(h->diag != 0.0, GH_ERR_ARG, "relax: zero diagonal");
+    GH_REQUIRE(f_dev != work_dev && s_dev != work_dev && f_dev != s_dev, GH_ERR_ARG,
+               "relax: f, s and work must be distinct");
+    hipStream_t st = (hipStream_t)stream;
+    // f' = (1 - w) f + (w / c0) (s - (L f - c0 f)) = -(w / c0) L f + f + (w / c0) s
+    const double a = -w / h->diag;
+    const void* cur = f_dev;
+    void* nxt = work_dev;
+    for (int n = 0; n < nrelax; n++) {
+        int rc = gh::stencil_apply_scaled(h, cur, nxt, 1, nullptr, nullptr, s_dev,
+                                          -a, a, 1.0, st);
+        if (rc != GH_OK) return rc;
+        void* done = nxt;
+        nxt = (void*)cur;
+        cur = done;
+    }
+    if (cur != f_dev) {
+        const size_t bytes = (size_t)(h->n[0] * h->n[1] * h->n[2]) * dtype_size(h->dtype);
+        GH_CHECK_HIP(hipMemcpyAsync(f_dev, cur, bytes, hipMemcpyDeviceToDevice, st));
+    }
+    return GH_OK;
 }
 
 }  // extern "C"

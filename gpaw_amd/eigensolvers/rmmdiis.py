@@ -221,21 +221,22 @@ class RMMDIIS:
         return lam
 
     def _diis_combine(self, wfs, psi_hist, R_hist, psib, Rb, nit):
-        """Per band: alpha minimising |sum_i alpha_i R_i| subject to
+        """Per band: alpha minimising |sum_i alpha_i R_i| with
         sum_i alpha_i = 1 over the nit + 1 stored iterates, then
         psi = sum_i alpha_i psi_i, R = sum_i alpha_i R_i
         (rmmdiis.py:200-239)."""
-        Rh = torch.stack([wfs.flat(r) for r in R_hist[:nit + 1]], 1)
-        Ph = torch.stack([wfs.flat(p) for p in psi_hist[:nit + 1]], 1)
-        nbk = Rh.shape[0]
-        # R_nn[b, i, j] = dv <R_i|R_j>
-        R_nn = wfs.dv * torch.bmm(Rh.conj(), Rh.transpose(1, 2).contiguous())
-        A = -torch.ones((nbk, nit + 2, nit + 2), dtype=R_nn.dtype,
-                        device='cuda')
-        A[:, :nit + 1, :nit + 1] = R_nn
+        k = nit + 1
+        # R_nn[b, i, j] = dv <R_i|R_j> (one pass over the k residuals)
+        R_nn = blas.multi_gram(R_hist[:k], scale=wfs.dv)
+        nbk = R_nn.shape[0]
+        A = -torch.ones((nbk, k + 1, k + 1), dtype=R_nn.dtype, device='cuda')
+        A[:, :k, :k] = R_nn
         A[:, -1, -1] = 0.0
-        x = torch.zeros((nbk, nit + 2, 1), dtype=R_nn.dtype, device='cuda')
+        x = torch.zeros((nbk, k + 1, 1), dtype=R_nn.dtype, device='cuda')
         x[:, -1, 0] = -1.0
-        alpha = torch.linalg.solve(A, x)[:, :-1, 0]
-        psib.copy_(torch.bmm(alpha[:, None, :], Ph).reshape(psib.shape))
-        Rb.copy_(torch.bmm(alpha[:, None, :], Rh).reshape(Rb.shape))
+        alpha = torch.linalg.solve(A, x)[:, :k, 0]
+        shape = (nbk,) + (1,) * (psib.dim() - 1)
+        for hist, out in ((psi_hist, psib), (R_hist, Rb)):
+            out.copy_(hist[nit] * alpha[:, nit].reshape(shape))
+            for i in range(nit):
+                out.add_(hist[i] * alpha[:, i].reshape(shape))

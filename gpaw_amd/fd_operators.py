@@ -9,10 +9,11 @@ Chelikowsky expansion coefficients).
 import ctypes
 
 import numpy as np
+import torch
 
 from . import _lib
-from ._arrays import (Staged, dtype_code, real_dtype, require_gpu,
-                      stream_ptr, torch_dtype)
+from ._arrays import (Staged, dtype_code, is_complex, real_dtype,
+                      require_gpu, stream_ptr, torch_dtype)
 
 # Expansion coefficients of the finite-difference Laplacian, J. R.
 # Chelikowsky et al., Phys. Rev. B 50, 11355 (1994) -- the table of
@@ -59,6 +60,7 @@ class FDOperator:
             self._keep[2][1], self._keep[3][1], dtype_code(dtype)))
         self.handle = h
         self._lib = L
+        self._work = None
         if description is None:
             description = '%d point finite-difference stencil' % self.npoints
         self.description = description
@@ -105,9 +107,24 @@ class FDOperator:
         return out_xR
 
     def relax(self, relax_method, f_g, s_g, n, w=None):
-        raise NotImplementedError(
-            'Gauss-Seidel/Jacobi relaxation belongs to the Poisson solver, '
-            'which this package does not provide')
+        """n relaxation sweeps of L f = s in place on f (fd_operators.py:126,
+        c/operators.c:55-95).  relax_method 2 (Jacobi) with weight w
+        (default 1); method 1 (Gauss-Seidel, a sequential sweep) raises
+        NotImplementedError."""
+        if relax_method != 2:
+            raise NotImplementedError(
+                'relax_method %r: only Jacobi (2) is provided on the GPU'
+                % (relax_method,))
+        assert tuple(f_g.shape) == self.shape == tuple(s_g.shape)
+        assert not is_complex(self.dtype)
+        w = 1.0 if w is None else float(w)
+        with Staged(f_g, writeback=True, dtype=self.dtype) as f, \
+                Staged(s_g, dtype=self.dtype) as s:
+            if self._work is None or self._work.shape != f.shape:
+                self._work = torch.empty_like(f)
+            _lib.check(self._lib.gh_stencil_relax(
+                self.handle, 2, f.data_ptr(), s.data_ptr(),
+                self._work.data_ptr(), int(n), w, stream_ptr()))
 
     def get_diagonal_element(self):
         return self._lib.gh_stencil_diagonal(self.handle)

@@ -84,6 +84,16 @@ int gh_stencil_apply(const gh_stencil* h, const void* in_dev, void* out_dev,
  * the LDS-ring kernel; 0 = automatic.  Used by the autotuner. */
 int gh_stencil_set_tile(gh_stencil* h, int rows, int planes, int depth);
 
+/* Jacobi relaxation of L f = s in place (replaces Operator.relax,
+ * c/operators.c:55-95 with c/bmgs/relax.c, relax_method 2): nrelax sweeps
+ *   f <- (1 - w) f + w (s - sum_{c != 0} coef_c f[shift_c]) / coef_0
+ * with the boundary conditions of gh_stencil_apply (no Bloch phases).
+ * One real grid (n0, n1, n2); work_dev: a device array of the same size.
+ * relax_method 1 (Gauss-Seidel) returns GH_ERR_UNSUPPORTED. */
+int gh_stencil_relax(const gh_stencil* h, int relax_method, void* f_dev,
+                     const void* s_dev, void* work_dev, int nrelax, double w,
+                     void* stream);
+
 /* -------------------------------------------------------- transformers --
  * Replaces cgpaw.Transformer (reference c/transformers.c:202-252, created
  * by gpaw/transformers.py:68-76) and its apply (transformers.c:118-164 ->
@@ -195,6 +205,19 @@ int gh_multi_axpy(int dtype, int64_t nbands, int64_t n, const double* a_dev,
 #define GH_DOT_CHUNKS 64
 int gh_multi_dotc(int dtype, int64_t nbands, int64_t n, const void* a_dev,
                   const void* b_dev, double scale, double* out_dev,
+                  double* work_dev, void* stream);
+
+/* Per-band Gram matrices of k vectors (the DIIS residual matrix R_nn of
+ * rmmdiis.py:207-212, wfs.integrate(R_i, R_j) = dv <R_i|R_j>):
+ *   out[x][i][j] = scale * sum_G conj(v_i[x, G]) v_j[x, G],  1 <= k <= 6,
+ * for x < nbands, each v_i a device array (nbands, n) of `dtype`.
+ * vecs_dev: host array of k device pointers.  out_dev: device
+ * (nbands, k, k) doubles for real dtypes, (nbands, k, k, 2) (re, im) for
+ * complex ones (Hermitian).  work_dev: gh_multi_gram_work(k, nbands)
+ * doubles.  FP64 accumulation in a fixed order. */
+int64_t gh_multi_gram_work(int k, int64_t nbands);
+int gh_multi_gram(int dtype, int64_t nbands, int64_t n, int k,
+                  const void* const* vecs_dev, double scale, double* out_dev,
                   double* work_dev, void* stream);
 
 #ifdef __cplusplus

@@ -86,7 +86,7 @@ def test_laplace_vs_oracle(N_c, h, pbc, n, dtype, k_c):
     assert relerr(host(out), ref) < tol
 
 
-@pytest.mark.parametrize('kernel', ['ring', 'reg'])
+@pytest.mark.parametrize('kernel', ['ring', 'reg', 'persist'])
 @pytest.mark.parametrize('N_c,pbc,dtype,k_c',
                          [((24, 20, 28), True, np.float64, None),
                           ((24, 20, 28), True, np.complex128, (0.25, 0, 0.1)),
@@ -95,11 +95,15 @@ def test_laplace_vs_oracle(N_c, h, pbc, n, dtype, k_c):
                           ((18, 14, 16), True, np.float32, None)])
 def test_stencil_kernel_variants(monkeypatch, kernel, N_c, pbc, dtype, k_c):
     """Both stencil kernels (LDS-ring DMA and register-queue) on the same
-    inputs, with partial last tiles (odd rows per tile) and short x-runs."""
+    inputs, with partial last tiles (odd rows per tile) and short x-runs;
+    'persist' runs the ring kernel as 8 resident blocks that walk several
+    tiles each."""
+    monkeypatch.delenv('GPAW_AMD_STENCIL', raising=False)
+    monkeypatch.delenv('GPAW_AMD_PERSIST', raising=False)
     if kernel == 'reg':
         monkeypatch.setenv('GPAW_AMD_STENCIL', 'reg')
-    else:
-        monkeypatch.delenv('GPAW_AMD_STENCIL', raising=False)
+    elif kernel == 'persist':
+        monkeypatch.setenv('GPAW_AMD_PERSIST', '8')
     rng = np.random.default_rng(12)
     gd = GridDescriptor(N_c, np.array(N_c) * 0.3, pbc)
     ogd = og.Grid(N_c, np.array(N_c) * 0.3, pbc)
@@ -354,3 +358,17 @@ def test_multi_dotc_and_axpy():
     y = dev(b.copy())
     blas.multi_axpy(dev(lam), dev(a), y)
     np.testing.assert_allclose(host(y), b + lam[:, None] * a, rtol=1e-14)
+
+
+@pytest.mark.parametrize('dtype', [np.float64, np.complex128])
+@pytest.mark.parametrize('k', [1, 3, 6])
+def test_multi_gram(dtype, k):
+    """Per-band Gram matrices of the DIIS step against numpy."""
+    from gpaw_amd import blas
+    rng = np.random.default_rng(13)
+    vs = [rand(rng, (5, 7, 9, 11), dtype) for _ in range(k)]
+    G = host(blas.multi_gram([dev(v) for v in vs], scale=0.3))
+    ref = 0.3 * np.einsum('ixg,jxg->xij',
+                          np.array([v.reshape(5, -1) for v in vs]).conj(),
+                          np.array([v.reshape(5, -1) for v in vs]))
+    np.testing.assert_allclose(G, ref, rtol=1e-13, atol=1e-13 * abs(ref).max())
