@@ -268,7 +268,10 @@ stencil_cfd_kernel(const V* __restrict__ in, V* __restrict__ out,
 // with vmcnt(0) just before the barrier of the iteration that first reads
 // it.  Compute wavefronts never write LDS: they refill the x register queue
 // from the ring, read y/z neighbours from the current plane's slot, apply
-// Bloch phases / zero ghosts at read time, and store H psi.
+// Bloch phases / zero ghosts at read time, and store H psi.  The potential
+// (and src) at their own points come from plain loads issued before the
+// LDS work of the plane: the potential is shared by every band and stays in
+// L2, and a DMA ring for it measured 15% slower than these loads.
 constexpr int kRingMaxThreads = 1024;
 constexpr int kDmaBytes = 16;              // bytes per lane per DMA op
 
@@ -318,8 +321,7 @@ __device__ __forceinline__ void ring_block_coords(const RingGeom& rg, int L,
 // (ty + 2R) rows, then D + 1 slots of ty rows for the potential (as S) and
 // D + 1 slots of ty rows for src, when present.
 struct RingLayout {
-    int psi_slot, aux_slot;     // elements per slot
-    int v_off, s_off;           // byte offsets of the potential / src rings
+    int psi_slot;               // elements per slot
 };
 
 template <typename V, typename S, int R, int PPT, bool HAS_V, bool HAS_SRC>
@@ -330,8 +332,6 @@ stencil_ring_kernel(const V* __restrict__ in, V* __restrict__ out,
                     RingLayout lay) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     V* ring = reinterpret_cast<V*>(smem_raw);
-    S* vring = reinterpret_cast<S*>(smem_raw + lay.v_off);
-    V* sring = reinterpret_cast<V*>(smem_raw + lay.s_off);
     const CfdGeom& g = rg.g;
     const int D = rg.nload;
     const int P = R + D + 1;
@@ -399,43 +399,7 @@ stencil_ring_kernel(const V* __restrict__ in, V* __restrict__ out,
                 }
             }
         };
-        // tile rows of plane xv of the potential and of src (no ghosts);
-        // plane xv is loaded by the owner of psi plane xv + R, together
-        // with it, into slot (xv - x0) % (D + 1)
-        const int aux_n = tye * n2;
-        auto issue_aux = [&](int xv) {
-            if (xv >= x1) return;
-            const int sl = (xv - x0) % (D + 1);
-            if (HAS_V) {
-                constexpr int EPLS = kDmaBytes / (int)sizeof(S);
-                const S* srcp = vt + (size_t)xv * plane + (size_t)y0 * n2;
-                S* dst = vring + sl * lay.aux_slot;
-                for (int off = 0; off < aux_n; off += 64 * EPLS) {
-                    const int e = off + lane * EPLS;
-                    if (e < aux_n)
-                        __builtin_amdgcn_global_load_lds(
-                            (const void*)(srcp + e),
-                            (__attribute__((address_space(3))) void*)(dst + off),
-                            kDmaBytes, 0, 0);
-                }
-            }
-            if (HAS_SRC) {
-                const V* srcp = src + band * bstride + (size_t)xv * plane + (size_t)y0 * n2;
-                V* dst = sring + sl * lay.aux_slot;
-                for (int off = 0; off < aux_n; off += CH) {
-                    const int e = off + lane * EPL;
-                    if (e < aux_n)
-                        __builtin_amdgcn_global_load_lds(
-                            (const void*)(srcp + e),
-                            (__attribute__((address_space(3))) void*)(dst + off),
-                            kDmaBytes, 0, 0);
-                }
-            }
-        };
-        for (int xp = x0 + w; xp < min(x0 + R + D, xend); xp += D) {
-            issue(xp);
-            if (xp - R >= x0) issue_aux(xp - R);
-        }
+        for (int xp = x0 + w; xp < min(x0 + R + D, xend); xp += D) issue(xp);
         for (int x = x0; x < x1; x++) {
             // plane x+R (queue refill) is first read now; the planes x of
             // the first R iterations were issued in the prologue and are
@@ -444,10 +408,7 @@ stencil_ring_kernel(const V* __restrict__ in, V* __restrict__ out,
                 wait_vmcnt0();
             ring_barrier();
             const int xp = x + R + D;
-            if (xp < xend && ((xp - x0) % D) == w) {
-                issue(xp);
-                issue_aux(xp - R);
-            }
+            if (xp < xend && ((xp - x0) % D) == w) issue(xp);
         }
         continue;
     }
@@ -507,7 +468,16 @@ stencil_ring_kernel(const V* __restrict__ in, V* __restrict__ out,
                 q[k][2 * R] = act[k] ? v : vzero<V>();
             }
         }
-        const int aux = ((x - x0) % (D + 1)) * lay.aux_slot;
+        // potential and src at the own points of plane x: plain loads
+        // (L2-resident for the potential), issued before the LDS work
+        S vv[PPT];
+        V ss[PPT];
+#pragma unroll
+        for (int k = 0; k < PPT; k++) {
+            const size_t gi = (size_t)x * plane + yz[k];
+            vv[k] = HAS_V ? vt[gi] : (S)0;
+            ss[k] = HAS_SRC ? src[band * bstride + gi] : vzero<V>();
+        }
         const V* sl = ring + ((x - x0) % P) * slot;
 #pragma unroll
         for (int k = 0; k < PPT; k++) {
@@ -541,9 +511,8 @@ stencil_ring_kernel(const V* __restrict__ in, V* __restrict__ out,
                 acc = fma_rs(cf.cm[2][j], vm, acc);
                 acc = fma_rs(cf.cp[2][j], vp, acc);
             }
-            const int own = yl[k] * n2 + z;
-            if (HAS_V) acc = fma_rs(vring[aux + own], q[k][R], acc);
-            if (HAS_SRC) acc = fma_rs(beta, sring[aux + own], acc);
+            if (HAS_V) acc = fma_rs(vv[k], q[k][R], acc);
+            if (HAS_SRC) acc = fma_rs(beta, ss[k], acc);
             if (rg.nt_store) store_nt(outb + (size_t)x * plane + yz[k], acc);
             else outb[(size_t)x * plane + yz[k]] = acc;
         }
@@ -762,16 +731,11 @@ int launch_ring_r(const gh_stencil* h, const void* in, void* out,
     const int n0 = (int)h->n[0], n1 = (int)h->n[1], n2 = (int)h->n[2];
     const int vs = (int)sizeof(V);
     if ((n2 * vs) % kDmaBytes != 0) return GH_ERR_UNSUPPORTED;
-    if (vt && (n2 * (int)sizeof(S)) % kDmaBytes != 0) return GH_ERR_UNSUPPORTED;
     auto lds_of = [&](int c, int d) {
-        size_t l = (size_t)(R + d + 1) * (c + 2 * R) * n2 * vs;
-        if (vt) l += (size_t)(d + 1) * c * n2 * sizeof(S);
-        if (src) l += (size_t)(d + 1) * c * n2 * vs;
-        return l;
+        return (size_t)(R + d + 1) * (c + 2 * R) * n2 * vs;
     };
-    // Largest tile (rows) that fits for each prefetch depth; the automatic
-    // choice prefers depth 3 unless that costs more than a quarter of the
-    // rows of depth 2.  Explicit rows/depth (autotuner) override.
+    // Largest tile (rows) that fits for each prefetch depth.  Explicit
+    // rows/depth (autotuner) override the automatic choice.
     auto max_rows = [&](int d) {
         int best = 0;
         for (int c = 1; c <= n1; c++) {
@@ -793,17 +757,16 @@ int launch_ring_r(const gh_stencil* h, const void* in, void* out,
         int mr = max_rows(D);
         ty = h->ty_req > 0 ? std::min(h->ty_req, mr) : divisor_rows(mr);
     } else {
-        int r3 = divisor_rows(max_rows(3)), r2 = divisor_rows(max_rows(2));
-        if (4 * r3 >= 3 * r2) { D = 3; ty = r3; } else { D = 2; ty = r2; }
-        if (h->ty_req > 0) ty = std::min(h->ty_req, max_rows(D));
+        // depth 2 with the largest tile: measured faster than depth 3 with
+        // fewer rows (256 x 112^3 FP64: 16 rows x 2 planes 1.84 ms,
+        // 14 rows x 3 planes 1.96 ms)
+        D = 2;
+        ty = h->ty_req > 0 ? std::min(h->ty_req, max_rows(D)) : divisor_rows(max_rows(D));
     }
     if (ty <= 0) return GH_ERR_UNSUPPORTED;
     const size_t lds = lds_of(ty, D);
     RingLayout lay;
     lay.psi_slot = (ty + 2 * R) * n2;
-    lay.aux_slot = ty * n2;
-    lay.v_off = (R + D + 1) * lay.psi_slot * vs;
-    lay.s_off = lay.v_off + (vt ? (D + 1) * lay.aux_slot * (int)sizeof(S) : 0);
     RingGeom rg;
     CfdGeom& g = rg.g;
     g.n0 = n0; g.n1 = n1; g.n2 = n2;
