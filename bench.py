@@ -29,6 +29,11 @@ sys.path.insert(0, ROOT)
 METRIC = ('band-Hψ/s at 112³ (GB/s vs 8 TB/s); RMM-DIIS time to '
           '1e-8, NN vs MG precond')
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of the fused stencil at the default workload (256
+# bands, 112^3, FP64), from rocprofv3 FETCH_SIZE / WRITE_SIZE runs of this
+# script scaled by a plain copy of known size (x2 / x1); the commands and
+# raw values are in profiles/full_hpsi_fp64_112cubed_256bands.md.
+STENCIL_TRAFFIC_BYTES = {(112, 256): 6.798e9}
 
 
 def parse():
@@ -236,7 +241,7 @@ def main():
         'roofline': {'bound': 'hbm', 'achieved': round(achieved, 1),
                      'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
                      'frac': round(achieved / HBM_PEAK_GBS, 4),
-                     'traffic': None,
+                     'traffic': STENCIL_TRAFFIC_BYTES.get((N, nb)),
                      'kernel': 'stencil_ring_kernel<double,R=3,V> '
                                '(T psi + V psi, one pass)',
                      'kernel_ms': round(stencil_ms, 4)},
