@@ -53,6 +53,9 @@ def parse():
                    help='RMM-DIIS sweeps (0: skip the eigensolver leg)')
     p.add_argument('--rmm-converge', type=int, default=64,
                    help='lowest bands whose residuals make up the error')
+    p.add_argument('--no-c3', action='store_true',
+                   help='skip the complex64 stencil / transformer / V-cycle '
+                        'leg')
     return p.parse_args()
 
 
@@ -162,6 +165,79 @@ def rmmdiis_leg(w, gd, pt, kin, H, nb, sweeps, nconv, tol=1e-8):
             'eps_lowest': float(wfs.eps[0])}
 
 
+def _time_ms(fn, reps=5):
+    import torch
+    fn()
+    torch.cuda.synchronize()
+    out = []
+    for _ in range(reps):
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        fn()
+        e1.record()
+        out.append((e0, e1))
+    torch.cuda.synchronize()
+    return float(np.median([a.elapsed_time(b) for a, b in out]))
+
+
+def c3_leg(gd, nb):
+    """Complex64 kernels on 256 complex residuals at 112^3 with the Bloch
+    phase of k = (0.25, 0, 0): 19-point stencil, nn=3 restrict and
+    interpolate, and the reference's V-cycle (nn=1 transformers, two coarse
+    levels), with the complex128 V-cycle on the same input for the error
+    of the single-precision form."""
+    import torch
+    from gpaw_amd.fd_operators import Laplace
+    from gpaw_amd.preconditioner import Preconditioner
+    from gpaw_amd.transformers import Transformer
+    N = int(np.prod(gd.n_c))
+    ph = gd.phase_cd((0.25, 0.0, 0.0))
+    g = torch.Generator(device='cuda')
+    g.manual_seed(1)
+    R = torch.randn((nb,) + tuple(gd.n_c), dtype=torch.complex64,
+                    device='cuda', generator=g)
+    out = torch.empty_like(R)
+    kin = Laplace(gd, -0.5, 3, np.complex64)
+    gd1 = gd.coarsen()
+    coarse = gd1.empty(nb, np.complex64)
+    rst = Transformer(gd, gd1, 3, np.complex64)
+    itp = Transformer(gd1, gd, 3, np.complex64)
+
+    class K:
+        phase_cd = ph
+    pc = Preconditioner(gd, kin, np.complex64)
+    res = {'dtype': 'c64 (FP32 arithmetic)', 'bands': nb, 'k_c': [0.25, 0, 0]}
+    s = 8
+    t = _time_ms(lambda: kin.apply(R, out, ph))
+    res['stencil_ms'] = round(t, 4)
+    res['stencil_GBps'] = round(nb * 2 * N * s / t / 1e6, 1)
+    t = _time_ms(lambda: rst.apply(R, coarse, ph))
+    res['restrict_nn3_ms'] = round(t, 4)
+    res['restrict_nn3_GBps'] = round(nb * (N + N // 8) * s / t / 1e6, 1)
+    t = _time_ms(lambda: itp.apply(coarse, out, ph))
+    res['interpolate_nn3_ms'] = round(t, 4)
+    res['interpolate_nn3_GBps'] = round(nb * (N + N // 8) * s / t / 1e6, 1)
+    t = _time_ms(lambda: pc(R, K, out=out))
+    res['vcycle_ms'] = round(t, 4)
+    res['vcycle_GBps_compulsory'] = round(nb * 2 * N * s / t / 1e6, 1)
+    del coarse
+    # complex128 reference form on the same residuals
+    nref = min(nb, 32)
+    R2 = R[:nref].to(torch.complex128)
+    kin2 = Laplace(gd, -0.5, 3, complex)
+    pc2 = Preconditioner(gd, kin2, complex)
+    out2 = torch.empty_like(R2)
+    t = _time_ms(lambda: pc2(R2, K, out=out2))
+    res['vcycle_c128_ms_per_band'] = round(t / nref, 5)
+    res['vcycle_c64_ms_per_band'] = round(res['vcycle_ms'] / nb, 5)
+    pc(R[:nref].contiguous(), K, out=out[:nref])
+    err = (out[:nref].to(torch.complex128) - out2).abs().max() / \
+        out2.abs().max()
+    res['vcycle_c64_relerr_vs_c128'] = float(err)
+    return res
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -208,9 +284,15 @@ def main():
                                            8) / nb
     full_gbs = value * full_bytes_per_band / 1e9
 
+    c3 = None
+    if not args.no_c3:
+        del out
+        out = None
+        c3 = c3_leg(gd, nb)
+
     rmm = None
     if args.rmm_sweeps > 0:
-        del out
+        out = None
         rmm = rmmdiis_leg(w, gd, pt, kin, H, nb, args.rmm_sweeps,
                           min(args.rmm_converge, nb))
 
@@ -246,6 +328,7 @@ def main():
                                '(T psi + V psi, one pass)',
                      'kernel_ms': round(stencil_ms, 4)},
         'rmmdiis': rmm,
+        'c3_single_precision': c3,
         'cpu_baseline': cpu,
     }
     print(json.dumps(line))

@@ -51,6 +51,7 @@ _SIGNATURES = {
     'gh_multi_axpy': (_c.c_int, [_c.c_int, I64, I64, P, P, P, P]),
     'gh_multi_dotc': (_c.c_int, [_c.c_int, I64, I64, P, P, D, P, P, P]),
     'gh_multi_gram_work': (I64, [_c.c_int, I64]),
+    'gh_add_to_density': (_c.c_int, [_c.c_int, I64, I64, P, P, P, P]),
     'gh_multi_gram': (_c.c_int, [_c.c_int, I64, I64, _c.c_int,
                                  _c.POINTER(P), D, P, P, P]),
     'gh_spline_coefs': (_c.c_int, [_c.c_int, D, PD, PD]),

@@ -204,6 +204,29 @@ int gram_k(int k, int64_t nbands, int64_t n, const void* const* vecs, double sca
     return GH_ERR_ARG;
 }
 
+// Pseudo-density accumulation (c/utilities.c:313-356, called per band by
+// wavefunctions/fd.py:95-100): nt[G] += sum_n f_n |psi_n(G)|^2.  One thread
+// per grid point sweeps the bands in order (the reference's order of the
+// per-band additions), FP64 accumulation for every data type.
+__device__ __forceinline__ double abs2(double a) { return a * a; }
+__device__ __forceinline__ double abs2(float a) { return (double)a * (double)a; }
+template <typename S>
+__device__ __forceinline__ double abs2(cplx<S> a) {
+    return (double)a.re * (double)a.re + (double)a.im * (double)a.im;
+}
+
+template <typename V>
+__global__ void __launch_bounds__(256)
+add_to_density_kernel(const double* __restrict__ f, const V* __restrict__ psit,
+                      double* __restrict__ nt, int64_t nbands, int64_t n) {
+    for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < n;
+         g += (int64_t)gridDim.x * blockDim.x) {
+        double acc = nt[g];
+        for (int64_t b = 0; b < nbands; b++) acc += f[b] * abs2(psit[b * n + g]);
+        nt[g] = acc;
+    }
+}
+
 template <typename V, typename S>
 int launch_axpbyz(double alpha, const void* x, double beta, const void* y,
                   void* z, int64_t total, hipStream_t st) {
@@ -300,6 +323,24 @@ int gh_multi_gram(int dtype, int64_t nbands, int64_t n, int k,
     }
     set_error("multi_gram: bad dtype");
     return GH_ERR_ARG;
+}
+
+int gh_add_to_density(int dtype, int64_t nbands, int64_t n, const double* f_dev,
+                      const void* psit_dev, double* nt_dev, void* stream) {
+    GH_REQUIRE(f_dev && psit_dev && nt_dev && n >= 0 && nbands >= 0, GH_ERR_ARG,
+               "add_to_density: bad argument");
+    if (n == 0 || nbands == 0) return GH_OK;
+    hipStream_t st = (hipStream_t)stream;
+    const int64_t nblk = std::min<int64_t>((n + 255) / 256, 16384);
+    switch (dtype) {
+        case GH_F64: hipLaunchKernelGGL((add_to_density_kernel<double>), dim3((unsigned)nblk), dim3(256), 0, st, f_dev, (const double*)psit_dev, nt_dev, nbands, n); break;
+        case GH_C128: hipLaunchKernelGGL((add_to_density_kernel<cplx<double>>), dim3((unsigned)nblk), dim3(256), 0, st, f_dev, (const cplx<double>*)psit_dev, nt_dev, nbands, n); break;
+        case GH_F32: hipLaunchKernelGGL((add_to_density_kernel<float>), dim3((unsigned)nblk), dim3(256), 0, st, f_dev, (const float*)psit_dev, nt_dev, nbands, n); break;
+        case GH_C64: hipLaunchKernelGGL((add_to_density_kernel<cplx<float>>), dim3((unsigned)nblk), dim3(256), 0, st, f_dev, (const cplx<float>*)psit_dev, nt_dev, nbands, n); break;
+        default: set_error("add_to_density: bad dtype"); return GH_ERR_ARG;
+    }
+    GH_CHECK_LAUNCH();
+    return GH_OK;
 }
 
 int64_t gh_precond_scratch_bytes(const gh_stencil* kin0, const gh_stencil* kin1,

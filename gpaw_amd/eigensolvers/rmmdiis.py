@@ -64,6 +64,11 @@ class FDWaveFunctions:
             S = S + ham.dS(P) @ P.conj().T
         return S
 
+    def add_to_density_from_k_point_with_occupation(self, nt_G, f_n):
+        """nt_G += sum_n f_n |psit_n|^2 (wavefunctions/fd.py:95-100)."""
+        from ..density import add_to_density_bands
+        return add_to_density_bands(f_n, self.psit, nt_G)
+
     def orthonormalize(self, ham=None):
         """psi <- L^-1 psi, P <- L^-1 P with L L^H = S' (fdpw.py:394-424)."""
         S = self.overlap(ham)

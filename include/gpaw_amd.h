@@ -215,6 +215,13 @@ int gh_multi_dotc(int dtype, int64_t nbands, int64_t n, const void* a_dev,
  * (nbands, k, k) doubles for real dtypes, (nbands, k, k, 2) (re, im) for
  * complex ones (Hermitian).  work_dev: gh_multi_gram_work(k, nbands)
  * doubles.  FP64 accumulation in a fixed order. */
+/* Pseudo-density accumulation, replaces the per-band calls of
+ * cgpaw.add_to_density (c/utilities.c:313-356; wavefunctions/fd.py:95-100):
+ *   nt[G] += sum_{x < nbands} f[x] |psit[x, G]|^2   (FP64 nt, any dtype)
+ * f_dev: nbands doubles on the device; bands are added in order. */
+int gh_add_to_density(int dtype, int64_t nbands, int64_t n, const double* f_dev,
+                      const void* psit_dev, double* nt_dev, void* stream);
+
 int64_t gh_multi_gram_work(int k, int64_t nbands);
 int gh_multi_gram(int dtype, int64_t nbands, int64_t n, int k,
                   const void* const* vecs_dev, double scale, double* out_dev,

@@ -18,9 +18,12 @@ spline      radial cubic spline (bmgs/spline.c:11-58)
 lfc         sphere boxes, spline_to_grid, LFC integrate / add
             (lfc.py:63-119, 273-374, 392-631; lfc.c:432-505, 818-977)
 precond     the 14-step multigrid V-cycle (preconditioner.py:35-93)
-dense       S/H subspace matrices, eigh, inverse Cholesky, rotation
-            (eigensolver.py:175-236, fdpw.py:394-424, matrix.py:229-327)
-rmmdiis     blocked RMM-DIIS iteration (rmmdiis.py:63-272)
+rmmdiis     Hamiltonian with dH/dS, subspace diagonalisation, inverse-Cholesky
+            orthonormalisation and the blocked RMM-DIIS sweep
+            (eigensolver.py:153-236, fdpw.py:394-424, rmmdiis.py:63-254)
+poisson     Jacobi relaxation and the multigrid Poisson solver
+            (c/bmgs/relax.c, c/operators.c:55-95, poisson.py:318-546)
+density     pseudo-density accumulation (c/utilities.c:313-356)
 refbuild    ctypes loader for ``oracle/_ref/libgpawref.so``, the reference C
             kernels compiled from their own sources by ``oracle/Makefile``
 

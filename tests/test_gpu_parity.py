@@ -220,7 +220,9 @@ def test_transformers_single(dtype):
 
 @pytest.mark.parametrize('dtype,k_c', [(np.float64, None),
                                        (np.complex128, (0.25, 0, 0)),
-                                       (np.complex128, (0.1, 0.2, -0.3))])
+                                       (np.complex128, (0.1, 0.2, -0.3)),
+                                       (np.float32, None),
+                                       (np.complex64, (0.25, 0, 0))])
 def test_precond_vs_oracle(dtype, k_c):
     rng = np.random.default_rng(7)
     N_c = np.array([24, 28, 32])
@@ -235,8 +237,10 @@ def test_precond_vs_oracle(dtype, k_c):
         phase_cd = None if k_c is None else ogd0.phase_cd(k_c)
     d = P(dev(R), K)
     c0, o0 = ost.laplace_stencil(ogd0.h_cv, -0.5, 3)
-    ref = opc.Preconditioner(ogd0, c0, o0)(R, K.phase_cd)
-    assert relerr(host(d), ref) < 1e-12
+    wide = np.complex128 if np.dtype(dtype).kind == 'c' else np.float64
+    ref = opc.Preconditioner(ogd0, c0, o0)(R.astype(wide), K.phase_cd)
+    single = dtype in (np.float32, np.complex64)
+    assert relerr(host(d), ref) < (2e-6 if single else 1e-12)
 
 
 def _lfc_pair(rng, N_c, h, natoms, dtype, k_c):
@@ -372,3 +376,25 @@ def test_multi_gram(dtype, k):
                           np.array([v.reshape(5, -1) for v in vs]).conj(),
                           np.array([v.reshape(5, -1) for v in vs]))
     np.testing.assert_allclose(G, ref, rtol=1e-13, atol=1e-13 * abs(ref).max())
+
+
+@pytest.mark.parametrize('dtype', [np.float64, np.complex128, np.float32,
+                                   np.complex64])
+def test_add_to_density(dtype):
+    """nt += sum_n f_n |psi_n|^2 against the oracle (c/utilities.c)."""
+    from gpaw_amd.density import add_to_density, add_to_density_bands
+    from oracle import density as ode
+    rng = np.random.default_rng(14)
+    psit = rand(rng, (7, 10, 12, 14), dtype)
+    f = rng.uniform(0, 2, 7)
+    nt0 = rng.standard_normal((10, 12, 14))
+    ref = ode.add_to_density(f, psit.astype(np.complex128), nt0.copy())
+    nt = dev(nt0)
+    add_to_density_bands(f, dev(psit), nt)
+    tol = 1e-6 if dtype in (np.float32, np.complex64) else 1e-14
+    assert relerr(host(nt), ref) < tol
+    # the per-band drop-in
+    nt1 = dev(nt0)
+    for fn, p in zip(f, psit):
+        add_to_density(fn, dev(p), nt1)
+    assert relerr(host(nt1), ref) < tol
