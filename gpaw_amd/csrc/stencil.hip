@@ -21,6 +21,7 @@
 // thread per point reading its neighbours through the cache hierarchy.
 #include <cstdarg>
 #include <cstdlib>
+#include <type_traits>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -324,7 +325,11 @@ struct RingLayout {
     int psi_slot;               // elements per slot
 };
 
-template <typename V, typename S, int R, int PPT, bool HAS_V, bool HAS_SRC>
+// SYM: the coefficients of +j and -j are equal along every axis (every
+// Laplacian); the paired real path then adds the two neighbours first and
+// keeps half the coefficients live.
+template <typename V, typename S, int R, int PPT, bool HAS_V, bool HAS_SRC,
+          bool SYM = false>
 __global__ void __launch_bounds__(kRingMaxThreads)
 stencil_ring_kernel(const V* __restrict__ in, V* __restrict__ out,
                     const S* __restrict__ vt, const V* __restrict__ src,
@@ -415,6 +420,148 @@ stencil_ring_kernel(const V* __restrict__ in, V* __restrict__ out,
 
     // ------------------------------------------------------ compute --
     V* __restrict__ outb = out + band * bstride;
+    if constexpr (!is_complex<V>::value && PPT == 2) {
+        // Real data: each thread owns two z-adjacent points (z, z + 1) of
+        // one row (n2 is even: rows are whole 16-byte multiples), so every
+        // LDS read, potential / src load and store moves both points at
+        // once.  Wrapped neighbours need no phase; those beyond a
+        // non-periodic face are zero.  The z-window z - 2P .. z + 1 + 2P is
+        // read as pairs whose index wraps modulo n2 / 2.
+        typedef V V2 __attribute__((ext_vector_type(2)));
+        const int np = n2 / 2;                       // pairs per row
+        const int pidx = threadIdx.x;                // < ncomp * 64
+        const bool act = pidx < tye * np;
+        const int pp = act ? pidx : 0;
+        const int yl = pp / np;
+        const int zp = pp - yl * np;                 // pair index in row
+        const int yz = (y0 + yl) * n2 + 2 * zp;
+        constexpr int NPW = (R + 1) / 2;             // window pairs per side
+        int zoff[2 * NPW];                           // wrapped pair index
+        bool zout[2 * NPW];                          // beyond a non-pbc face
+#pragma unroll
+        for (int i = 0; i < NPW; i++) {
+            int zm = zp - 1 - i, zq = zp + 1 + i;
+            zout[i] = zm < 0 && !g.pbc2;
+            zout[NPW + i] = zq >= np && !g.pbc2;
+            zoff[i] = zm < 0 ? zm + np : zm;
+            zoff[NPW + i] = zq >= np ? zq - np : zq;
+        }
+        // rows of the tile beyond a non-periodic y face are never written by
+        // the loaders: zero them once in every slot, so the loop reads them
+        // without a test (the first barrier of the loop orders this)
+        if (!g.pbc1) {
+            for (int r = 0; r < 2 * R; r++) {
+                const int lrow = r < R ? r : tye + r;
+                const int y = y0 - R + lrow;
+                if (y >= 0 && y < g.n1) continue;
+                for (int sl = 0; sl < P; sl++)
+                    for (int e = pidx; e < n2; e += rg.ncomp * 64)
+                        ring[sl * slot + lrow * n2 + e] = (V)0;
+            }
+        }
+        V2 q[2 * R + 1];
+#pragma unroll
+        for (int j = 0; j < 2 * R; j++) {
+            q[j].x = load_x(inb, x0 - R + j, yz, g, ph);
+            q[j].y = load_x(inb, x0 - R + j, yz + 1, g, ph);
+        }
+        // the coefficients live in VGPRs (uniform values; the scalar file
+        // is full in this loop and would spill them through v_readlane)
+        S k0 = cf.c0;
+        S km[3][R], kp[3][R];
+        asm volatile("" : "+v"(k0));
+#pragma unroll
+        for (int c = 0; c < 3; c++)
+#pragma unroll
+            for (int j = 0; j < R; j++) {
+                km[c][j] = cf.cm[c][j];
+                kp[c][j] = SYM ? km[c][j] : cf.cp[c][j];
+                asm volatile("" : "+v"(km[c][j]));
+                if (!SYM) asm volatile("" : "+v"(kp[c][j]));
+            }
+        auto march = [&](auto zper) {
+            constexpr bool ZPER = decltype(zper)::value;
+            for (int x = x0; x < x1; x++) {
+                ring_barrier();
+                {
+                    const int xp = x + R;
+                    const V2* sl = reinterpret_cast<const V2*>(ring + ((xp - x0) % P) * slot);
+                    V2 v = sl[(yl + R) * np + zp];
+                    if (xp >= g.n0 && !g.pbc0) v = V2{(V)0, (V)0};
+                    q[2 * R] = v;
+                }
+                const size_t gi = (size_t)x * plane + yz;
+                V2 vv, ss;
+                if (HAS_V) vv = *reinterpret_cast<const V2*>(vt + gi);
+                if (HAS_SRC) ss = *reinterpret_cast<const V2*>(src + band * bstride + gi);
+                const V2* sl = reinterpret_cast<const V2*>(ring + ((x - x0) % P) * slot);
+                V2 acc = q[R] * k0;
+#pragma unroll
+                for (int j = 0; j < R; j++) {
+                    if constexpr (SYM) {
+                        acc = (q[R - 1 - j] + q[R + 1 + j]) * km[0][j] + acc;
+                    } else {
+                        acc = q[R - 1 - j] * km[0][j] + acc;
+                        acc = q[R + 1 + j] * kp[0][j] + acc;
+                    }
+                }
+                const int row = yl + R;
+#pragma unroll
+                for (int j = 0; j < R; j++) {
+                    const V2 m = sl[(row - 1 - j) * np + zp];
+                    const V2 pv = sl[(row + 1 + j) * np + zp];
+                    if constexpr (SYM) {
+                        acc = (m + pv) * km[1][j] + acc;
+                    } else {
+                        acc = m * km[1][j] + acc;
+                        acc = pv * kp[1][j] + acc;
+                    }
+                }
+                // z-window: w[0] = pair zp - NPW ... w[2 NPW] = pair zp + NPW
+                V2 w[2 * NPW + 1];
+                w[NPW] = q[R];
+                const V2* brow = sl + row * np;
+#pragma unroll
+                for (int i = 0; i < NPW; i++) {
+                    V2 lo = brow[zoff[i]], hi = brow[zoff[NPW + i]];
+                    if constexpr (!ZPER) {
+                        if (zout[i]) lo = V2{(V)0, (V)0};
+                        if (zout[NPW + i]) hi = V2{(V)0, (V)0};
+                    }
+                    w[NPW - 1 - i] = lo;
+                    w[NPW + 1 + i] = hi;
+                }
+                // element e of the window: pair e >> 1, lane e & 1; element
+                // 2 NPW (pair NPW, lane 0) is the point z
+                auto we = [&](int e) { return (e & 1) ? w[e >> 1].y : w[e >> 1].x; };
+#pragma unroll
+                for (int j = 0; j < R; j++) {
+                    const int c = 2 * NPW;
+                    if constexpr (SYM) {
+                        acc.x = fma_rs(km[2][j], we(c - 1 - j) + we(c + 1 + j), acc.x);
+                        acc.y = fma_rs(km[2][j], we(c - j) + we(c + 2 + j), acc.y);
+                    } else {
+                        acc.x = fma_rs(km[2][j], we(c - 1 - j), acc.x);
+                        acc.x = fma_rs(kp[2][j], we(c + 1 + j), acc.x);
+                        acc.y = fma_rs(km[2][j], we(c - j), acc.y);
+                        acc.y = fma_rs(kp[2][j], we(c + 2 + j), acc.y);
+                    }
+                }
+                if (HAS_V) acc = vv * q[R] + acc;
+                if (HAS_SRC) acc = ss * beta + acc;
+                if (act) {
+                    V2* o = reinterpret_cast<V2*>(outb + gi);
+                    if (rg.nt_store) __builtin_nontemporal_store(acc, o);
+                    else *o = acc;
+                }
+#pragma unroll
+                for (int j = 0; j < 2 * R; j++) q[j] = q[j + 1];
+            }
+        };
+        if (g.pbc2) march(std::true_type{});
+        else march(std::false_type{});
+        continue;
+    }
     const int NT = rg.ncomp * 64;
     const int tile_pts = tye * n2;
     int yl[PPT], zz[PPT], yz[PPT];
@@ -820,19 +967,34 @@ int launch_ring_r(const gh_stencil* h, const void* in, void* out,
     dim3 grid((unsigned)nblocks), block((rg.ncomp + rg.nload) * 64);
     const S* vtp = (const S*)vt;
     const V* srcp = (const V*)src;
-#define GH_LAUNCH_RING(HV, HS)                                                 \
+    // symmetric coefficients: the paired real path keeps half of them live
+    bool sym = true;
+    for (int c = 0; c < 3; c++)
+        for (int j = 0; j < R; j++) sym = sym && h->cm[c][j] == h->cp[c][j];
+    constexpr bool kPaired = !is_complex<V>::value && PPT == 2;
+#define GH_LAUNCH_RING_S(HV, HS, SY)                                           \
     do {                                                                       \
-        auto kfn = stencil_ring_kernel<V, S, R, PPT, HV, HS>;                  \
+        auto kfn = stencil_ring_kernel<V, S, R, PPT, HV, HS, SY>;              \
         GH_CHECK_HIP(hipFuncSetAttribute((const void*)kfn,                     \
             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));            \
         hipLaunchKernelGGL(kfn, grid, block, lds, st, (const V*)in, (V*)out,   \
                            vtp, srcp, (S)beta, cf, rg, ph, lay);               \
+    } while (0)
+#define GH_LAUNCH_RING(HV, HS)                                                 \
+    do {                                                                       \
+        if constexpr (kPaired) {                                               \
+            if (sym) GH_LAUNCH_RING_S(HV, HS, true);                           \
+            else GH_LAUNCH_RING_S(HV, HS, false);                              \
+        } else {                                                               \
+            GH_LAUNCH_RING_S(HV, HS, false);                                   \
+        }                                                                      \
     } while (0)
     if (vt && src) GH_LAUNCH_RING(true, true);
     else if (vt) GH_LAUNCH_RING(true, false);
     else if (src) GH_LAUNCH_RING(false, true);
     else GH_LAUNCH_RING(false, false);
 #undef GH_LAUNCH_RING
+#undef GH_LAUNCH_RING_S
     GH_CHECK_LAUNCH();
     return GH_OK;
 }
