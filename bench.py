@@ -33,7 +33,7 @@ HBM_PEAK_GBS = 8000.0
 # bands, 112^3, FP64), from rocprofv3 FETCH_SIZE / WRITE_SIZE runs of this
 # script scaled by a plain copy of known size (x2 / x1); the commands and
 # raw values are in profiles/full_hpsi_fp64_112cubed_256bands.md.
-STENCIL_TRAFFIC_BYTES = {(112, 256): 6.798e9}
+STENCIL_TRAFFIC_BYTES = {(112, 256): 6.808e9}
 
 
 def parse():
