@@ -154,6 +154,21 @@ def rmmdiis_leg(w, gd, pt, kin, H, nb, sweeps, nconv, tol=1e-8):
             break
     torch.cuda.synchronize()
     t = time.perf_counter() - t0
+    # the dense contractions of a sweep, timed alone (rocBLAS FP64 MFMA)
+    a = wfs.psit.reshape(nb, -1)
+    b = torch.empty_like(wfs.psit).reshape(nb, -1)
+    b.copy_(a)
+    U = torch.linalg.qr(torch.randn((nb, nb), dtype=torch.float64,
+                                    device='cuda'))[0]
+    t_h = _time_ms(lambda: a @ b.T)
+    t_r = _time_ms(lambda: U @ a)
+    flop = 2.0 * nb * nb * a.shape[1]
+    dense = {'H_matrix_ms': round(t_h, 3),
+             'H_matrix_TFLOPs': round(flop / t_h / 1e9, 1),
+             'rotation_ms': round(t_r, 3),
+             'rotation_TFLOPs': round(flop / t_r / 1e9, 1),
+             'peak_fp64_mfma_TFLOPs': 78.6}
+    del b
     return {'precond': 'multigrid V-cycle, FP64',
             'start': 'atom-centred s/p Gaussians, orthonormalised',
             'bands': nb, 'bands_converged': nconv,
@@ -162,7 +177,8 @@ def rmmdiis_leg(w, gd, pt, kin, H, nb, sweeps, nconv, tol=1e-8):
             'err_first': errs[0] if errs else None,
             'err_last': errs[-1] if errs else None,
             'seconds_to_1e-8': None if t_tol is None else round(t_tol, 3),
-            'eps_lowest': float(wfs.eps[0])}
+            'eps_lowest': float(wfs.eps[0]),
+            'dense_fp64': dense}
 
 
 def _time_ms(fn, reps=5):

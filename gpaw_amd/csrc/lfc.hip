@@ -35,7 +35,7 @@ namespace gh {
 constexpr int kLfcMaxNm = 64;   // functions per atom supported
 constexpr int kIntChunk = 256;  // entries staged per integrate step
 constexpr int kAddBT = 32;      // bands per add wavefront
-constexpr int kAddGrp = 8;      // bands loaded together in add
+constexpr int kAddGrp = 32;     // bands loaded together in add
 constexpr int kAddChunk = 64;   // entries per add wavefront (one per lane)
 
 template <typename V> struct IntBT { static constexpr int value = 16; };

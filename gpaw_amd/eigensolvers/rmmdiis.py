@@ -1,7 +1,9 @@
 """Blocked RMM-DIIS on the GPU (gpaw/eigensolvers/rmmdiis.py counterpart).
 
 FDWaveFunctions holds one k-point's wave functions psit (nbands, n0, n1, n2)
-on the device with eigenvalues eps and projections P = <p|psit>.
+on the device with eigenvalues eps and projections P = <p|psit>.  Rotations
+write a new buffer and re-bind wfs.psit (the reference copies back into
+psit); read wfs.psit after an iteration rather than keeping the tensor.
 RMMDIIS.iterate restates Eigensolver.iterate + RMMDIIS.iterate_one_k_point
 (eigensolver.py:121-147, rmmdiis.py:63-254):
 
@@ -76,7 +78,9 @@ class FDWaveFunctions:
         Linv = torch.linalg.solve_triangular(
             L, torch.eye(len(S), dtype=S.dtype, device=S.device), upper=False)
         psi = self.flat(self.psit)
-        self.psit.copy_((Linv @ psi).reshape(self.psit.shape))
+        # the rotated wave functions go to a new buffer (no copy back):
+        # psit is re-bound, as is every rotation below
+        self.psit = (Linv @ psi).reshape(self.psit.shape)
         if self.P is not None:
             self.P = Linv @ self.P
         self.orthonormalized = True
@@ -129,8 +133,8 @@ class RMMDIIS:
         H = 0.5 * (H + H.conj().T)
         eps, C = torch.linalg.eigh(H)
         U = C.T.contiguous()
-        psit.copy_((U @ psi).reshape(psit.shape))
-        Htpsit.copy_((U @ hpsi).reshape(psit.shape))
+        wfs.psit = (U @ psi).reshape(psit.shape)
+        Htpsit = (U @ hpsi).reshape(psit.shape)
         if P is not None:
             wfs.P = U @ P
         wfs.eps = eps.to(torch.float64).contiguous()
@@ -242,6 +246,6 @@ class RMMDIIS:
         alpha = torch.linalg.solve(A, x)[:, :k, 0]
         shape = (nbk,) + (1,) * (psib.dim() - 1)
         for hist, out in ((psi_hist, psib), (R_hist, Rb)):
-            out.copy_(hist[nit] * alpha[:, nit].reshape(shape))
+            torch.mul(hist[nit], alpha[:, nit].reshape(shape), out=out)
             for i in range(nit):
-                out.add_(hist[i] * alpha[:, i].reshape(shape))
+                out.addcmul_(hist[i], alpha[:, i].reshape(shape))
