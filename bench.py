@@ -128,8 +128,8 @@ def cpu_baseline(w, gd_args, vt, nb_sample):
 
 
 def rmmdiis_leg(w, gd, pt, kin, H, nb, sweeps, nconv, tol=1e-8):
-    """Blocked RMM-DIIS (B=10, niter=3) with the multigrid preconditioner
-    on the same Hamiltonian, from the atom-centred start.  Error as the
+    """RMM-DIIS (niter=3, all bands in one block) with the multigrid
+    preconditioner on the same Hamiltonian, from the atom-centred start.  Error as the
     reference's (sum_n w_n int |R_n|^2 dv) with w_n = 1 for the lowest
     `nconv` bands, divided by nconv.  Returns timing and convergence."""
     import torch

@@ -919,8 +919,21 @@ int launch_ring_r(const gh_stencil* h, const void* in, void* out,
     g.n0 = n0; g.n1 = n1; g.n2 = n2;
     g.pbc0 = h->pbc[0]; g.pbc1 = h->pbc[1]; g.pbc2 = h->pbc[2];
     g.ty = ty;
-    g.lx = h->lx_req > 0 ? std::min(h->lx_req, n0) : n0;
     g.nyt = (n1 + ty - 1) / ty;
+    if (h->lx_req > 0) {
+        g.lx = std::min(h->lx_req, n0);
+    } else {
+        // one block per CU: when the bands x y-tiles do not cover the CUs,
+        // split the x-march into chunks (each chunk re-reads 2R planes),
+        // keeping chunks at least 4R planes long
+        int ncu = 0, dev = 0;
+        GH_CHECK_HIP(hipGetDevice(&dev));
+        GH_CHECK_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+        const int64_t base = nbands * (int64_t)g.nyt;
+        int nxc = 1;
+        while (base * nxc < ncu && (n0 + nxc) / (nxc + 1) >= 4 * R) nxc++;
+        g.lx = (n0 + nxc - 1) / nxc;
+    }
     g.nxc = (n0 + g.lx - 1) / g.lx;
     rg.nload = D;
     rg.ncomp = ((ty * n2 + PPT - 1) / PPT + 63) / 64;

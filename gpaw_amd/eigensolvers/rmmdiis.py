@@ -92,9 +92,15 @@ class _Kpt:
 
 
 class RMMDIIS:
-    """RMM-DIIS eigensolver (rmmdiis.py:10-50 parameters)."""
+    """RMM-DIIS eigensolver (rmmdiis.py:10-50 parameters).
 
-    def __init__(self, keep_htpsit=True, blocksize=10, niter=3, rtol=1e-16,
+    blocksize: bands updated together.  Every step of the iteration is per
+    band, so the result does not depend on it (the reference's own
+    blocked-vs-unblocked test, test_blocked_rmm_diis.py); None (default)
+    takes all bands at once, which fills the GPU (the reference's FD default
+    is 10, sized for CPU caches)."""
+
+    def __init__(self, keep_htpsit=True, blocksize=None, niter=3, rtol=1e-16,
                  limit_lambda=False, trial_step=0.1):
         if not keep_htpsit:
             raise NotImplementedError('keep_htpsit=False')
@@ -179,7 +185,7 @@ class RMMDIIS:
         psit = wfs.psit
         self.calculate_residuals(ham, wfs, psit, wfs.P, wfs.eps, R)
         kpt = _Kpt(wfs.phase_cd)
-        B = self.blocksize
+        B = self.blocksize or nb
         errors = []
         dR_full = None
         for n1 in range(0, nb, B):

@@ -128,3 +128,24 @@ def test_rmmdiis_converges_to_dense(k_c):
     errors = [es.iterate(ham, wfs) for _ in range(30)]
     assert errors[-1] < 1e-6 * errors[0]
     assert abs(host(wfs.eps)[:4] - w[:4]).max() < 1e-10
+
+
+def test_blocked_vs_unblocked():
+    """test_blocked_rmm_diis.py: the block size does not change the
+    iteration (every step is per band)."""
+    ham_o, _, rng = small_system(N=12, h=0.4, seed=5)
+    gd, ham, pc = gpu_system(ham_o)
+    Hm, Sm = dense_pair(ham_o)
+    nb = 6
+    w, W = sl.eigh(Hm, Sm, subset_by_index=[0, nb - 1])
+    psi = start_guess(ham_o, rng, nb, W, 0.3)
+    out = []
+    for B in (2, None):
+        wfs = FDWaveFunctions(gd, dev(psi), ham.pt, pc, None)
+        es = RMMDIIS(blocksize=B)
+        errs = [es.iterate(ham, wfs) for _ in range(3)]
+        out.append((errs, host(wfs.eps), host(wfs.psit)))
+    (e1, eps1, p1), (e2, eps2, p2) = out
+    assert np.allclose(e1, e2, rtol=1e-12, atol=0)
+    assert abs(eps1 - eps2).max() < 1e-12
+    assert abs(p1 - p2).max() < 1e-11 * abs(p2).max()
