@@ -296,6 +296,7 @@ struct RingGeom {
     int nbands;
     int nlogical;         // logical tiles: nbands * nyt * nxc
     int rounds;           // logical tiles per block (persistent launch)
+    int cplx_fast;        // complex data, trivial y / z Bloch phases
 };
 
 // Block -> (band, x chunk, y tile).  Logical ids of one XCD are contiguous
@@ -467,17 +468,18 @@ stencil_ring_kernel(const V* __restrict__ in, V* __restrict__ out,
         }
         // the coefficients live in VGPRs (uniform values; the scalar file
         // is full in this loop and would spill them through v_readlane)
+        // (symmetric stencils only: with both signs pinned the register
+        // file overflows)
         S k0 = cf.c0;
         S km[3][R], kp[3][R];
-        asm volatile("" : "+v"(k0));
+        if (SYM) asm volatile("" : "+v"(k0));
 #pragma unroll
         for (int c = 0; c < 3; c++)
 #pragma unroll
             for (int j = 0; j < R; j++) {
                 km[c][j] = cf.cm[c][j];
                 kp[c][j] = SYM ? km[c][j] : cf.cp[c][j];
-                asm volatile("" : "+v"(km[c][j]));
-                if (!SYM) asm volatile("" : "+v"(kp[c][j]));
+                if (SYM) asm volatile("" : "+v"(km[c][j]));
             }
         auto march = [&](auto zper) {
             constexpr bool ZPER = decltype(zper)::value;
@@ -562,6 +564,148 @@ stencil_ring_kernel(const V* __restrict__ in, V* __restrict__ out,
         else march(std::false_type{});
         continue;
     }
+    if constexpr (is_complex<V>::value && PPT == 2) {
+      {
+        // Complex data whose y and z Bloch phases are 1: the stencil's real
+        // coefficients act on real and imaginary parts alike, so this is the
+        // paired path above on (re, im) pairs -- a thread owns two
+        // z-adjacent points as one 4-vector (re0, im0, re1, im1).  Only the
+        // x-wrap phase (uniform per plane) is applied, at the queue refill.
+        typedef S V4 __attribute__((ext_vector_type(4)));
+        typedef S S2 __attribute__((ext_vector_type(2)));
+        const int np = n2 / 2;
+        const int pidx = threadIdx.x;
+        const bool act = pidx < tye * np;
+        const int pp = act ? pidx : 0;
+        const int yl = pp / np;
+        const int zp = pp - yl * np;
+        const int yz = (y0 + yl) * n2 + 2 * zp;
+        constexpr int NPW = (R + 1) / 2;
+        int zoff[2 * NPW];
+        bool zout[2 * NPW];
+#pragma unroll
+        for (int i = 0; i < NPW; i++) {
+            int zm = zp - 1 - i, zq = zp + 1 + i;
+            zout[i] = zm < 0 && !g.pbc2;
+            zout[NPW + i] = zq >= np && !g.pbc2;
+            zoff[i] = zm < 0 ? zm + np : zm;
+            zoff[NPW + i] = zq >= np ? zq - np : zq;
+        }
+        if (!g.pbc1) {
+            for (int r = 0; r < 2 * R; r++) {
+                const int lrow = r < R ? r : tye + r;
+                const int y = y0 - R + lrow;
+                if (y >= 0 && y < g.n1) continue;
+                for (int sl = 0; sl < P; sl++)
+                    for (int e = pidx; e < n2; e += rg.ncomp * 64)
+                        ring[sl * slot + lrow * n2 + e] = vzero<V>();
+            }
+        }
+        // (symmetric stencils only: with both signs pinned the register
+        // file overflows)
+        S k0 = cf.c0;
+        S km[3][R], kp[3][R];
+        if (SYM) asm volatile("" : "+v"(k0));
+#pragma unroll
+        for (int c = 0; c < 3; c++)
+#pragma unroll
+            for (int j = 0; j < R; j++) {
+                km[c][j] = cf.cm[c][j];
+                kp[c][j] = SYM ? km[c][j] : cf.cp[c][j];
+                if (SYM) asm volatile("" : "+v"(km[c][j]));
+            }
+        auto pack = [&](V a, V b) { return V4{a.re, a.im, b.re, b.im}; };
+        V4 q[2 * R + 1];
+#pragma unroll
+        for (int j = 0; j < 2 * R; j++)
+            q[j] = pack(load_x(inb, x0 - R + j, yz, g, ph),
+                        load_x(inb, x0 - R + j, yz + 1, g, ph));
+        const S pr = ph.re[0][0], pi = ph.im[0][0];   // x-wrap high -> low
+        for (int x = x0; x < x1; x++) {
+            ring_barrier();
+            {
+                const int xp = x + R;
+                const V4* sl = reinterpret_cast<const V4*>(ring + ((xp - x0) % P) * slot);
+                V4 v = sl[(yl + R) * np + zp];
+                if (xp >= g.n0) {
+                    if (!g.pbc0) v = V4{0, 0, 0, 0};
+                    else v = V4{v.x * pr - v.y * pi, v.x * pi + v.y * pr,
+                                v.z * pr - v.w * pi, v.z * pi + v.w * pr};
+                }
+                q[2 * R] = v;
+            }
+            const size_t gi = (size_t)x * plane + yz;
+            S2 vv;
+            V4 ss;
+            if (HAS_V) vv = *reinterpret_cast<const S2*>(vt + gi);
+            if (HAS_SRC) ss = *reinterpret_cast<const V4*>(src + band * bstride + gi);
+            const V4* sl = reinterpret_cast<const V4*>(ring + ((x - x0) % P) * slot);
+            V4 acc = q[R] * k0;
+#pragma unroll
+            for (int j = 0; j < R; j++) {
+                if constexpr (SYM) {
+                    acc = (q[R - 1 - j] + q[R + 1 + j]) * km[0][j] + acc;
+                } else {
+                    acc = q[R - 1 - j] * km[0][j] + acc;
+                    acc = q[R + 1 + j] * kp[0][j] + acc;
+                }
+            }
+            const int row = yl + R;
+#pragma unroll
+            for (int j = 0; j < R; j++) {
+                const V4 m = sl[(row - 1 - j) * np + zp];
+                const V4 pv = sl[(row + 1 + j) * np + zp];
+                if constexpr (SYM) {
+                    acc = (m + pv) * km[1][j] + acc;
+                } else {
+                    acc = m * km[1][j] + acc;
+                    acc = pv * kp[1][j] + acc;
+                }
+            }
+            V4 w[2 * NPW + 1];
+            w[NPW] = q[R];
+            const V4* brow = sl + row * np;
+#pragma unroll
+            for (int i = 0; i < NPW; i++) {
+                V4 lo = brow[zoff[i]], hi = brow[zoff[NPW + i]];
+                if (zout[i]) lo = V4{0, 0, 0, 0};
+                if (zout[NPW + i]) hi = V4{0, 0, 0, 0};
+                w[NPW - 1 - i] = lo;
+                w[NPW + 1 + i] = hi;
+            }
+            // complex element e of the window (pair e >> 1, point e & 1)
+            auto we = [&](int e) {
+                return (e & 1) ? S2{w[e >> 1].z, w[e >> 1].w} : S2{w[e >> 1].x, w[e >> 1].y};
+            };
+            S2 a0 = S2{acc.x, acc.y}, a1 = S2{acc.z, acc.w};
+#pragma unroll
+            for (int j = 0; j < R; j++) {
+                const int c = 2 * NPW;
+                if constexpr (SYM) {
+                    a0 = (we(c - 1 - j) + we(c + 1 + j)) * km[2][j] + a0;
+                    a1 = (we(c - j) + we(c + 2 + j)) * km[2][j] + a1;
+                } else {
+                    a0 = we(c - 1 - j) * km[2][j] + a0;
+                    a0 = we(c + 1 + j) * kp[2][j] + a0;
+                    a1 = we(c - j) * km[2][j] + a1;
+                    a1 = we(c + 2 + j) * kp[2][j] + a1;
+                }
+            }
+            acc = V4{a0.x, a0.y, a1.x, a1.y};
+            if (HAS_V) acc = V4{vv.x, vv.x, vv.y, vv.y} * q[R] + acc;
+            if (HAS_SRC) acc = ss * beta + acc;
+            if (act) {
+                V4* o = reinterpret_cast<V4*>(outb + gi);
+                if (rg.nt_store) __builtin_nontemporal_store(acc, o);
+                else *o = acc;
+            }
+#pragma unroll
+            for (int j = 0; j < 2 * R; j++) q[j] = q[j + 1];
+        }
+        continue;
+      }
+    } else {
+    // general path: complex data with Bloch phases (one point per thread)
     const int NT = rg.ncomp * 64;
     const int tile_pts = tye * n2;
     int yl[PPT], zz[PPT], yz[PPT];
@@ -668,6 +812,7 @@ stencil_ring_kernel(const V* __restrict__ in, V* __restrict__ out,
 #pragma unroll
             for (int j = 0; j < 2 * R; j++) q[k][j] = q[k][j + 1];
     }
+    }  // general path
     }  // rounds
 }
 
@@ -977,6 +1122,14 @@ int launch_ring_r(const gh_stencil* h, const void* in, void* out,
             cf.cp[c][j] = (S)(alpha * h->cp[c][j]);
         }
     PhaseArgs<S> ph = phase_args<S>(phase_cd);
+    rg.cplx_fast = 0;
+    if (is_complex<V>::value) {
+        bool triv = true;
+        for (int c = 1; c < 3; c++)
+            for (int d = 0; d < 2; d++)
+                triv = triv && ph.re[c][d] == (S)1 && ph.im[c][d] == (S)0;
+        rg.cplx_fast = triv ? 1 : 0;
+    }
     dim3 grid((unsigned)nblocks), block((rg.ncomp + rg.nload) * 64);
     const S* vtp = (const S*)vt;
     const V* srcp = (const V*)src;
@@ -984,7 +1137,7 @@ int launch_ring_r(const gh_stencil* h, const void* in, void* out,
     bool sym = true;
     for (int c = 0; c < 3; c++)
         for (int j = 0; j < R; j++) sym = sym && h->cm[c][j] == h->cp[c][j];
-    constexpr bool kPaired = !is_complex<V>::value && PPT == 2;
+    constexpr bool kPaired = PPT == 2;
 #define GH_LAUNCH_RING_S(HV, HS, SY)                                           \
     do {                                                                       \
         auto kfn = stencil_ring_kernel<V, S, R, PPT, HV, HS, SY>;              \
@@ -1016,7 +1169,7 @@ int launch_ring_r(const gh_stencil* h, const void* in, void* out,
 template <typename V> struct RingPPT { static constexpr int value = 2; };
 template <> struct RingPPT<cplx<double>> { static constexpr int value = 1; };
 template <> struct RingPPT<float> { static constexpr int value = 2; };
-template <> struct RingPPT<cplx<float>> { static constexpr int value = 1; };
+template <> struct RingPPT<cplx<float>> { static constexpr int value = 2; };
 
 template <typename V, typename S, int PPT>
 int launch_cfd(const gh_stencil* h, const void* in, void* out, int64_t nb,
@@ -1025,10 +1178,25 @@ int launch_cfd(const gh_stencil* h, const void* in, void* out, int64_t nb,
     if (!h->reg_only) {
         constexpr int RP = RingPPT<V>::value;
         int rc = GH_ERR_UNSUPPORTED;
-        switch (h->R) {
-            case 1: rc = launch_ring_r<V, S, 1, RP>(h, in, out, nb, ph, vt, src, beta, al, ga, st); break;
-            case 2: rc = launch_ring_r<V, S, 2, RP>(h, in, out, nb, ph, vt, src, beta, al, ga, st); break;
-            case 3: rc = launch_ring_r<V, S, 3, RP>(h, in, out, nb, ph, vt, src, beta, al, ga, st); break;
+        // complex data takes the two-point path only when the y and z Bloch
+        // phases are 1; otherwise the one-point path with phases
+        bool two = RP == 2;
+        if (is_complex<V>::value && ph)
+            for (int c = 1; c < 3; c++)
+                for (int d = 0; d < 2; d++)
+                    two = two && ph[(c * 2 + d) * 2] == 1.0 && ph[(c * 2 + d) * 2 + 1] == 0.0;
+        if (two) {
+            switch (h->R) {
+                case 1: rc = launch_ring_r<V, S, 1, RP>(h, in, out, nb, ph, vt, src, beta, al, ga, st); break;
+                case 2: rc = launch_ring_r<V, S, 2, RP>(h, in, out, nb, ph, vt, src, beta, al, ga, st); break;
+                case 3: rc = launch_ring_r<V, S, 3, RP>(h, in, out, nb, ph, vt, src, beta, al, ga, st); break;
+            }
+        } else {
+            switch (h->R) {
+                case 1: rc = launch_ring_r<V, S, 1, 1>(h, in, out, nb, ph, vt, src, beta, al, ga, st); break;
+                case 2: rc = launch_ring_r<V, S, 2, 1>(h, in, out, nb, ph, vt, src, beta, al, ga, st); break;
+                case 3: rc = launch_ring_r<V, S, 3, 1>(h, in, out, nb, ph, vt, src, beta, al, ga, st); break;
+            }
         }
         if (rc != GH_ERR_UNSUPPORTED) return rc;
     }

@@ -65,6 +65,12 @@ STENCIL_CASES = [
     ((10, 12, 200), 0.3, True, 4, np.float64, None),
     ((20, 24, 28), 0.35, True, 3, np.float32, None),
     ((20, 24, 28), 0.35, True, 3, np.complex64, (0.25, 0, 0)),
+    ((20, 24, 28), 0.35, True, 3, np.complex64, (0.1, -0.3, 0.45)),
+    ((16, 18, 20), 0.35, (True, False, True), 3, np.complex64,
+     (0.25, 0, 0)),
+    ((16, 18, 20), 0.35, (False, True, False), 2, np.complex64,
+     (0, 0.2, 0)),
+    ((18, 16, 22), 0.35, (True, True, False), 3, np.float32, None),
 ]
 
 
