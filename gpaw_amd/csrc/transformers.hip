@@ -66,24 +66,73 @@ __device__ __forceinline__ bool wrap_idx(int& i, int n, int pbc, int ax,
     return true;
 }
 
-// Stage input rows [iy0, iy0 + nrows) of input plane ix (full z) into A.
+// Register prefetch of a staged plane: thread t holds elements
+// t + s * blockDim.x (s < SPT) of the nrows x n2 stage, loaded while the
+// previous plane is filtered; the part of larger tiles beyond
+// SPT * blockDim.x is staged synchronously.
+template <typename V> struct StageRegs {
+    // real data only: for complex data the extra registers cost more than
+    // the prefetch gains (measured at 112^3 -> 56^3)
+    static constexpr int SPT = is_complex<V>::value ? 0 : 4;
+    V v[SPT > 0 ? SPT : 1];
+};
+
 template <typename V, typename S>
-__device__ __forceinline__ void stage_rows(const V* __restrict__ inb, int ix,
-                                           int iy0, int nrows, V* A,
-                                           const TGeom& g,
+__device__ __forceinline__ void stage_load(const V* __restrict__ inb, int ix,
+                                           int iy0, int nrows,
+                                           StageRegs<V>& R, const TGeom& g,
                                            const TPhase<S>& ph) {
     const int n2 = g.n[2];
+    const int tot = nrows * n2;
     S xre = 1, xim = 0;
     int x = ix;
-    bool xok = wrap_idx(x, g.n[0], g.pbc[0], 0, ph, xre, xim);
-    const V* pl = inb + (size_t)x * g.n[1] * n2;
-    for (int i = threadIdx.x; i < nrows * n2; i += blockDim.x) {
-        int r = i / n2;
-        int z = i - r * n2;
-        int y = iy0 + r;
-        S pre = xre, pim = xim;
-        bool ok = xok && wrap_idx(y, g.n[1], g.pbc[1], 1, ph, pre, pim);
-        A[i] = ok ? mul_phase(pl[y * n2 + z], pre, pim) : vzero<V>();
+    // the plane after the last one a block uses may lie beyond one period:
+    // nothing is loaded for it
+    bool xok = wrap_idx(x, g.n[0], g.pbc[0], 0, ph, xre, xim) && x >= 0 && x < g.n[0];
+    const V* pl = inb + (size_t)(xok ? x : 0) * g.n[1] * n2;
+#pragma unroll
+    for (int s = 0; s < StageRegs<V>::SPT; s++) {
+        const int i = threadIdx.x + s * blockDim.x;
+        V v = vzero<V>();
+        if (i < tot && xok) {
+            int r = i / n2;
+            int z = i - r * n2;
+            int y = iy0 + r;
+            S pre = xre, pim = xim;
+            if (wrap_idx(y, g.n[1], g.pbc[1], 1, ph, pre, pim))
+                v = mul_phase(pl[y * n2 + z], pre, pim);
+        }
+        R.v[s] = v;
+    }
+}
+
+template <typename V, typename S>
+__device__ __forceinline__ void stage_store(const V* __restrict__ inb, int ix,
+                                            int iy0, int nrows,
+                                            const StageRegs<V>& R, V* A,
+                                            const TGeom& g,
+                                            const TPhase<S>& ph) {
+    const int n2 = g.n[2];
+    const int tot = nrows * n2;
+#pragma unroll
+    for (int s = 0; s < StageRegs<V>::SPT; s++) {
+        const int i = threadIdx.x + s * blockDim.x;
+        if (i < tot) A[i] = R.v[s];
+    }
+    const int done = StageRegs<V>::SPT * blockDim.x;
+    if (tot > done) {
+        S xre = 1, xim = 0;
+        int x = ix;
+        bool xok = wrap_idx(x, g.n[0], g.pbc[0], 0, ph, xre, xim);
+        const V* pl = inb + (size_t)x * g.n[1] * n2;
+        for (int i = done + threadIdx.x; i < tot; i += blockDim.x) {
+            int r = i / n2;
+            int z = i - r * n2;
+            int y = iy0 + r;
+            S pre = xre, pim = xim;
+            bool ok = xok && wrap_idx(y, g.n[1], g.pbc[1], 1, ph, pre, pim);
+            A[i] = ok ? mul_phase(pl[y * n2 + z], pre, pim) : vzero<V>();
+        }
     }
 }
 
@@ -150,9 +199,13 @@ restrict_kernel(const V* __restrict__ in, V* __restrict__ out, TGeom g,
     V q[PPT][QL];
     const int base = 2 * I0 - g.plo[0];  // first input plane of output I0
 
+    // plane ix is in registers on entry (loaded during the previous
+    // plane's filters); the next plane ix + 1 is loaded during this one's
+    StageRegs<V> pre;
     auto process_plane = [&](int ix, int slot) {
-        stage_rows<V, S>(inb, ix, iy0, nrows, A, g, ph);
+        stage_store<V, S>(inb, ix, iy0, nrows, pre, A, g, ph);
         __syncthreads();
+        stage_load<V, S>(inb, ix + 1, iy0, nrows, pre, g, ph);
         // z-filter every staged row
         for (int i = threadIdx.x; i < nrows * m2; i += blockDim.x) {
             int r = i / m2;
@@ -179,6 +232,7 @@ restrict_kernel(const V* __restrict__ in, V* __restrict__ out, TGeom g,
     };
 
     // prime the first 2K-3 planes
+    stage_load<V, S>(inb, base, iy0, nrows, pre, g, ph);
 #pragma unroll
     for (int s = 0; s < QL - 2; s++) process_plane(base + s, s);
     for (int I = I0; I < I1; I++) {
@@ -261,9 +315,12 @@ interpolate_kernel(const V* __restrict__ in, V* __restrict__ out, TGeom g,
     }
     V q[PPT][K];  // coarse planes c - K + 1 .. c after the y/z filters
 
+    // plane ix is in registers on entry; ix + 1 is loaded during its filters
+    StageRegs<V> pre;
     auto process_plane = [&](int ix, int slot) {
-        stage_rows<V, S>(inb, ix, cy0, nrows, A, g, ph);
+        stage_store<V, S>(inb, ix, cy0, nrows, pre, A, g, ph);
         __syncthreads();
+        stage_load<V, S>(inb, ix + 1, cy0, nrows, pre, g, ph);
         for (int i = threadIdx.x; i < nrows * m2; i += blockDim.x) {
             int r = i / m2;
             int kk = i - r * m2;
@@ -297,6 +354,7 @@ interpolate_kernel(const V* __restrict__ in, V* __restrict__ out, TGeom g,
     const int i_first = (O0 + g.skip[0]) >> 1;
     const int i_last = (O1 - 1 + g.skip[0]) >> 1;
     const int cbase = nn - 1 - g.plo[0];
+    stage_load<V, S>(inb, i_first + cbase - nn + 1, cy0, nrows, pre, g, ph);
 #pragma unroll
     for (int s = 0; s < K - 1; s++) process_plane(i_first + cbase - nn + 1 + s, s);
     for (int i = i_first; i <= i_last; i++) {
