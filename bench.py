@@ -127,17 +127,20 @@ def cpu_baseline(w, gd_args, vt, nb_sample):
                        '(%.2f s each)' % (nb_sample, len(times), t))}
 
 
-def rmmdiis_leg(w, gd, pt, kin, H, nb, sweeps, nconv, tol=1e-8):
+def rmmdiis_leg(w, gd, pt, kin, H, nb, sweeps, nconv, tol=1e-8,
+                precision=None, dense=True):
     """RMM-DIIS (niter=3, all bands in one block) with the multigrid
-    preconditioner on the same Hamiltonian, from the atom-centred start.  Error as the
-    reference's (sum_n w_n int |R_n|^2 dv) with w_n = 1 for the lowest
-    `nconv` bands, divided by nconv.  Returns timing and convergence."""
+    preconditioner (FP64, or FP32 with precision='single') on the same
+    Hamiltonian, from the atom-centred start.  Error as the reference's
+    (sum_n w_n int |R_n|^2 dv) with w_n = 1 for the lowest `nconv` bands,
+    divided by nconv.  Returns timing and convergence."""
     import torch
     from gpaw_amd import synthetic as syn
     from gpaw_amd.eigensolvers import RMMDIIS, FDWaveFunctions
     from gpaw_amd.preconditioner import Preconditioner
     psi0 = syn.initial_guess(gd, w.pos_av, nb)
-    wfs = FDWaveFunctions(gd, psi0, pt, Preconditioner(gd, kin))
+    wfs = FDWaveFunctions(gd, psi0, pt,
+                          Preconditioner(gd, kin, precision=precision))
     es = RMMDIIS()
     weights = torch.zeros(nb, dtype=torch.float64, device='cuda')
     weights[:nconv] = 1.0
@@ -154,6 +157,19 @@ def rmmdiis_leg(w, gd, pt, kin, H, nb, sweeps, nconv, tol=1e-8):
             break
     torch.cuda.synchronize()
     t = time.perf_counter() - t0
+    res = {'precond': 'multigrid V-cycle, %s' % (
+               'FP32 (residuals, subspace and Cholesky FP64)'
+               if precision == 'single' else 'FP64'),
+           'start': 'atom-centred s/p Gaussians, orthonormalised',
+           'bands': nb, 'bands_converged': nconv,
+           'sweeps': len(errs), 'seconds': round(t, 3),
+           'ms_per_sweep': round(1e3 * t / max(len(errs), 1), 2),
+           'err_first': errs[0] if errs else None,
+           'err_last': errs[-1] if errs else None,
+           'seconds_to_1e-8': None if t_tol is None else round(t_tol, 3),
+           'eps_lowest': float(wfs.eps[0])}
+    if not dense:
+        return res
     # the dense contractions of a sweep, timed alone (rocBLAS FP64 MFMA)
     a = wfs.psit.reshape(nb, -1)
     b = torch.empty_like(wfs.psit).reshape(nb, -1)
@@ -163,22 +179,14 @@ def rmmdiis_leg(w, gd, pt, kin, H, nb, sweeps, nconv, tol=1e-8):
     t_h = _time_ms(lambda: a @ b.T)
     t_r = _time_ms(lambda: U @ a)
     flop = 2.0 * nb * nb * a.shape[1]
-    dense = {'H_matrix_ms': round(t_h, 3),
-             'H_matrix_TFLOPs': round(flop / t_h / 1e9, 1),
-             'rotation_ms': round(t_r, 3),
-             'rotation_TFLOPs': round(flop / t_r / 1e9, 1),
-             'peak_fp64_mfma_TFLOPs': 78.6}
+    dense_ = {'H_matrix_ms': round(t_h, 3),
+              'H_matrix_TFLOPs': round(flop / t_h / 1e9, 1),
+              'rotation_ms': round(t_r, 3),
+              'rotation_TFLOPs': round(flop / t_r / 1e9, 1),
+              'peak_fp64_mfma_TFLOPs': 78.6}
     del b
-    return {'precond': 'multigrid V-cycle, FP64',
-            'start': 'atom-centred s/p Gaussians, orthonormalised',
-            'bands': nb, 'bands_converged': nconv,
-            'sweeps': len(errs), 'seconds': round(t, 3),
-            'ms_per_sweep': round(1e3 * t / max(len(errs), 1), 2),
-            'err_first': errs[0] if errs else None,
-            'err_last': errs[-1] if errs else None,
-            'seconds_to_1e-8': None if t_tol is None else round(t_tol, 3),
-            'eps_lowest': float(wfs.eps[0]),
-            'dense_fp64': dense}
+    res['dense_fp64'] = dense_
+    return res
 
 
 def _time_ms(fn, reps=5):
@@ -311,6 +319,9 @@ def main():
         out = None
         rmm = rmmdiis_leg(w, gd, pt, kin, H, nb, args.rmm_sweeps,
                           min(args.rmm_converge, nb))
+        rmm['single_precision_vcycle'] = rmmdiis_leg(
+            w, gd, pt, kin, H, nb, args.rmm_sweeps,
+            min(args.rmm_converge, nb), precision='single', dense=False)
 
     cpu = None
     if not args.no_cpu:

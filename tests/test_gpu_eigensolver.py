@@ -115,10 +115,15 @@ def test_rmmdiis_sweep_vs_oracle(k_c):
     assert abs(align(out, psi_o) - psi_o).max() < 1e-10 * abs(psi_o).max()
 
 
+@pytest.mark.parametrize('precision', [None, 'single'])
 @pytest.mark.parametrize('k_c', CASES)
-def test_rmmdiis_converges_to_dense(k_c):
+def test_rmmdiis_converges_to_dense(k_c, precision):
+    """Also with the single-precision V-cycle: residuals, subspace algebra
+    and orthonormalisation stay FP64, so the same eigenvalues are reached."""
     ham_o, _, rng = small_system(N=12, h=0.4, k_c=k_c, seed=4)
     gd, ham, pc = gpu_system(ham_o)
+    if precision:
+        pc = Preconditioner(gd, ham.kin, ham.dtype, precision=precision)
     Hm, Sm = dense_pair(ham_o)
     nb = 6
     w, W = sl.eigh(Hm, Sm, subset_by_index=[0, nb - 1])

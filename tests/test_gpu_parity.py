@@ -225,6 +225,29 @@ def test_transformers_single(dtype):
 
 
 @pytest.mark.parametrize('dtype,k_c', [(np.float64, None),
+                                       (np.complex128, (0.25, 0.1, 0))])
+def test_precond_single_precision_option(dtype, k_c):
+    """precision='single': FP64 residuals in and out, the cycle in
+    FP32 / complex64 (an extension); agrees with the FP64 oracle to single
+    precision."""
+    rng = np.random.default_rng(17)
+    N_c = np.array([24, 28, 32])
+    gd0 = GridDescriptor(N_c, N_c * 0.3)
+    ogd0 = og.Grid(N_c, N_c * 0.3)
+    kin0 = Laplace(gd0, -0.5, 3, dtype)
+    P = Preconditioner(gd0, kin0, dtype, precision='single')
+    R = rand(rng, (4,) + tuple(N_c), dtype)
+
+    class K:
+        phase_cd = None if k_c is None else ogd0.phase_cd(k_c)
+    d = P(dev(R), K)
+    assert d.dtype == dev(R).dtype
+    c0, o0 = ost.laplace_stencil(ogd0.h_cv, -0.5, 3)
+    ref = opc.Preconditioner(ogd0, c0, o0)(R, K.phase_cd)
+    assert relerr(host(d), ref) < 2e-6
+
+
+@pytest.mark.parametrize('dtype,k_c', [(np.float64, None),
                                        (np.complex128, (0.25, 0, 0)),
                                        (np.complex128, (0.1, 0.2, -0.3)),
                                        (np.float32, None),
