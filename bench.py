@@ -51,8 +51,12 @@ def parse():
                         '(known bytes) to calibrate HBM counters')
     p.add_argument('--rmm-sweeps', type=int, default=30,
                    help='RMM-DIIS sweeps (0: skip the eigensolver leg)')
+    p.add_argument('--dav-iters', type=int, default=30,
+                   help='Davidson iterates (0: skip the Davidson leg)')
     p.add_argument('--rmm-converge', type=int, default=64,
                    help='lowest bands whose residuals make up the error')
+    p.add_argument('--no-transformers', action='store_true',
+                   help='skip the restrict / interpolate leg')
     p.add_argument('--no-c3', action='store_true',
                    help='skip the complex64 stencil / transformer / V-cycle '
                         'leg')
@@ -128,20 +132,21 @@ def cpu_baseline(w, gd_args, vt, nb_sample):
 
 
 def rmmdiis_leg(w, gd, pt, kin, H, nb, sweeps, nconv, tol=1e-8,
-                precision=None, dense=True):
-    """RMM-DIIS (niter=3, all bands in one block) with the multigrid
-    preconditioner (FP64, or FP32 with precision='single') on the same
-    Hamiltonian, from the atom-centred start.  Error as the reference's
-    (sum_n w_n int |R_n|^2 dv) with w_n = 1 for the lowest `nconv` bands,
-    divided by nconv.  Returns timing and convergence."""
+                precision=None, dense=True, solver='rmm-diis'):
+    """RMM-DIIS (niter=3, all bands in one block) or Davidson (niter=2)
+    with the multigrid preconditioner (FP64, or FP32 with
+    precision='single') on the same Hamiltonian, from the atom-centred
+    start.  Error as the reference's (sum_n w_n int |R_n|^2 dv) with w_n = 1
+    for the lowest `nconv` bands, divided by nconv.  Returns timing and
+    convergence ('sweeps' counts Eigensolver.iterate calls)."""
     import torch
     from gpaw_amd import synthetic as syn
-    from gpaw_amd.eigensolvers import RMMDIIS, FDWaveFunctions
+    from gpaw_amd.eigensolvers import Davidson, RMMDIIS, FDWaveFunctions
     from gpaw_amd.preconditioner import Preconditioner
     psi0 = syn.initial_guess(gd, w.pos_av, nb)
     wfs = FDWaveFunctions(gd, psi0, pt,
                           Preconditioner(gd, kin, precision=precision))
-    es = RMMDIIS()
+    es = RMMDIIS() if solver == 'rmm-diis' else Davidson()
     weights = torch.zeros(nb, dtype=torch.float64, device='cuda')
     weights[:nconv] = 1.0
     torch.cuda.synchronize()
@@ -157,7 +162,9 @@ def rmmdiis_leg(w, gd, pt, kin, H, nb, sweeps, nconv, tol=1e-8,
             break
     torch.cuda.synchronize()
     t = time.perf_counter() - t0
-    res = {'precond': 'multigrid V-cycle, %s' % (
+    res = {'solver': repr(es) if solver != 'rmm-diis' else
+           'RMMDIIS(niter=3, all bands in one block)',
+           'precond': 'multigrid V-cycle, %s' % (
                'FP32 (residuals, subspace and Cholesky FP64)'
                if precision == 'single' else 'FP64'),
            'start': 'atom-centred s/p Gaussians, orthonormalised',
@@ -262,6 +269,39 @@ def c3_leg(gd, nb):
     return res
 
 
+def transformer_leg(gd, nb):
+    """Restriction 112^3 -> 56^3 and interpolation back, 256 bands, for
+    nn = 1 (the V-cycle's) and nn = 3 (density / potential) in float64,
+    complex64 and complex128 (complex with the Bloch phase of
+    k = (0.25, 0, 0)).  GB/s of the algorithmic bytes (N + N/8) * s per
+    band."""
+    import torch
+    from gpaw_amd.transformers import Transformer
+    N = int(np.prod(gd.n_c))
+    gd1 = gd.coarsen()
+    out = {}
+    for name, dt, tdt in (('f64', np.float64, torch.float64),
+                          ('c64', np.complex64, torch.complex64),
+                          ('c128', np.complex128, torch.complex128)):
+        ph = None if dt == np.float64 else gd.phase_cd((0.25, 0.0, 0.0))
+        fine = torch.randn((nb,) + tuple(gd.n_c), dtype=tdt, device='cuda')
+        coarse = gd1.empty(nb, dt)
+        s = fine.element_size()
+        for nn in (1, 3):
+            rst = Transformer(gd, gd1, nn, dt)
+            itp = Transformer(gd1, gd, nn, dt)
+            tr = _time_ms(lambda: rst.apply(fine, coarse, ph))
+            ti = _time_ms(lambda: itp.apply(coarse, fine, ph))
+            byt = nb * (N + N // 8) * s
+            out['%s_nn%d' % (name, nn)] = {
+                'restrict_ms': round(tr, 4),
+                'restrict_GBps': round(byt / tr / 1e6, 1),
+                'interpolate_ms': round(ti, 4),
+                'interpolate_GBps': round(byt / ti / 1e6, 1)}
+        del fine, coarse
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -314,6 +354,11 @@ def main():
         out = None
         c3 = c3_leg(gd, nb)
 
+    trans = None
+    if not args.no_transformers:
+        out = None
+        trans = transformer_leg(gd, nb)
+
     rmm = None
     if args.rmm_sweeps > 0:
         out = None
@@ -322,6 +367,13 @@ def main():
         rmm['single_precision_vcycle'] = rmmdiis_leg(
             w, gd, pt, kin, H, nb, args.rmm_sweeps,
             min(args.rmm_converge, nb), precision='single', dense=False)
+
+    dav = None
+    if args.dav_iters > 0:
+        out = None
+        dav = rmmdiis_leg(w, gd, pt, kin, H, nb, args.dav_iters,
+                          min(args.rmm_converge, nb), dense=False,
+                          solver='davidson')
 
     cpu = None
     if not args.no_cpu:
@@ -355,7 +407,9 @@ def main():
                                '(T psi + V psi, one pass)',
                      'kernel_ms': round(stencil_ms, 4)},
         'rmmdiis': rmm,
+        'davidson': dav,
         'c3_single_precision': c3,
+        'transformers': trans,
         'cpu_baseline': cpu,
     }
     print(json.dumps(line))

@@ -54,6 +54,8 @@ _SIGNATURES = {
     'gh_add_to_density': (_c.c_int, [_c.c_int, I64, I64, P, P, P, P]),
     'gh_multi_gram': (_c.c_int, [_c.c_int, I64, I64, _c.c_int,
                                  _c.POINTER(P), D, P, P, P]),
+    'gh_multi_lincomb': (_c.c_int, [_c.c_int, I64, I64, _c.c_int,
+                                    _c.POINTER(P), P, P, P]),
     'gh_spline_coefs': (_c.c_int, [_c.c_int, D, PD, PD]),
     'gh_spline_to_grid': (I64, [_c.c_int, _c.c_int, D, PD, PI64, PI64, PD,
                                 PD, PI64, PI64, PD, PI32]),

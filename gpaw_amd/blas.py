@@ -5,6 +5,8 @@ multi_dotc(a_xG, b_xG, scale): Re sum conj(a[x]) b[x] * scale, FP64 result
 with FP64 accumulation and a fixed reduction order.
 multi_gram(vecs, scale): G[x, i, j] = scale * sum conj(v_i[x]) v_j[x] for a
 list of k <= 6 arrays of equal shape (one pass over the data).
+multi_lincomb(vecs, coef, out): out[x] = sum_i coef[x, i] vecs[i][x] for
+k <= 8 arrays (one pass; `out` may be one of the inputs).
 """
 import numpy as np
 import torch
@@ -67,3 +69,33 @@ def multi_gram(vecs, scale=1.0):
         dtype_code(v0.dtype), nb, n, k, ptrs, float(scale), out.data_ptr(),
         work.data_ptr(), stream_ptr()))
     return torch.view_as_complex(out) if cplx else out
+
+
+def multi_lincomb(vecs, coef, out=None):
+    """out[x] = sum_i coef[x, i] vecs[i][x], terms accumulated in the order
+    given (rmmdiis.py:226-237; with coef = (1, lam) the out-of-place
+    per-band axpy).  coef: (nbands, k) real or complex; complex
+    coefficients need complex data."""
+    k = len(vecs)
+    v0 = vecs[0]
+    for v in vecs:
+        assert v.shape == v0.shape and v.dtype == v0.dtype
+        assert v.is_contiguous() and v.is_cuda
+    nb, n = _nb_n(v0)
+    if out is None:
+        out = torch.empty_like(v0)
+    assert out.shape == v0.shape and out.dtype == v0.dtype
+    assert out.is_contiguous()
+    coef = torch.as_tensor(coef, device='cuda')
+    assert coef.shape == (nb, k), coef.shape
+    if v0.is_complex():
+        c = torch.view_as_real(coef.to(torch.complex128).contiguous())
+    else:
+        assert not coef.is_complex(), 'complex coefficients for real data'
+        c = coef.to(torch.float64)
+    c = c.contiguous()
+    ptrs = (_lib.P * k)(*[v.data_ptr() for v in vecs])
+    _lib.check(_lib.lib().gh_multi_lincomb(
+        dtype_code(v0.dtype), nb, n, k, ptrs, c.data_ptr(), out.data_ptr(),
+        stream_ptr()))
+    return out

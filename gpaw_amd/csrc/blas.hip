@@ -49,6 +49,103 @@ __global__ void axpbyz_kernel(S alpha, const V* __restrict__ x, S beta,
         z[i] = vadd(vscale(x[i], alpha), vscale(y[i], beta));
 }
 
+// out[x] = sum_i c[x, i] v_i[x] per band x: the DIIS combination
+// psi = alpha_nit psi_nit + sum_{i<nit} alpha_i psi_i of the reference
+// (eigensolvers/rmmdiis.py:226-237) and out-of-place axpy (v_0 + lam v_1) in
+// one pass.  Terms are accumulated in the order given.  Coefficients are
+// FP64 (real data) or FP64 (re, im) pairs (complex data).  `out` may be one
+// of the inputs: every element is read before it is written, by the same
+// thread.
+constexpr int kLincombMax = 8;
+template <typename V> struct LincombPtrs { const V* v[kLincombMax]; };
+
+template <typename S>
+__device__ __forceinline__ S coef_mul(S v, S cr, S) { return cr * v; }
+template <typename S>
+__device__ __forceinline__ cplx<S> coef_mul(cplx<S> v, S cr, S ci) {
+    return cplx<S>{cr * v.re - ci * v.im, cr * v.im + ci * v.re};
+}
+template <typename S>
+__device__ __forceinline__ S coef_fma(S v, S cr, S, S acc) { return fma(cr, v, acc); }
+template <typename S>
+__device__ __forceinline__ cplx<S> coef_fma(cplx<S> v, S cr, S ci, cplx<S> acc) {
+    return cplx<S>{fma(cr, v.re, fma(-ci, v.im, acc.re)),
+                   fma(cr, v.im, fma(ci, v.re, acc.im))};
+}
+
+template <typename V, typename S, int K, int EPT>
+__global__ void __launch_bounds__(256)
+multi_lincomb_kernel(LincombPtrs<V> p, const double* __restrict__ coef, V* out,
+                     int64_t n) {
+    constexpr bool CPLX = is_complex<V>::value;
+    const int64_t band = blockIdx.y;
+    S cr[K], ci[K];
+#pragma unroll
+    for (int i = 0; i < K; i++) {
+        if (CPLX) {
+            cr[i] = (S)coef[2 * (band * K + i)];
+            ci[i] = (S)coef[2 * (band * K + i) + 1];
+        } else {
+            cr[i] = (S)coef[band * K + i];
+            ci[i] = 0;
+        }
+    }
+    const size_t off = (size_t)band * n;
+    const int64_t step = (int64_t)gridDim.x * blockDim.x * EPT;
+    for (int64_t i0 = (int64_t)blockIdx.x * blockDim.x * EPT + threadIdx.x; i0 < n;
+         i0 += step) {
+        V x[EPT][K];
+#pragma unroll
+        for (int e = 0; e < EPT; e++) {
+            const int64_t idx = i0 + (int64_t)e * blockDim.x;
+#pragma unroll
+            for (int i = 0; i < K; i++)
+                x[e][i] = idx < n ? p.v[i][off + idx] : vzero<V>();
+        }
+#pragma unroll
+        for (int e = 0; e < EPT; e++) {
+            const int64_t idx = i0 + (int64_t)e * blockDim.x;
+            V acc = coef_mul(x[e][0], cr[0], ci[0]);
+#pragma unroll
+            for (int i = 1; i < K; i++) acc = coef_fma(x[e][i], cr[i], ci[i], acc);
+            if (idx < n) out[off + idx] = acc;
+        }
+    }
+}
+
+template <typename V, typename S, int K>
+void launch_lincomb(int64_t nb, int64_t n, const void* const* vecs,
+                    const double* coef, void* out, hipStream_t st) {
+    constexpr int EPT = K <= 2 ? 4 : 2;
+    LincombPtrs<V> p{};
+    for (int i = 0; i < K; i++) p.v[i] = (const V*)vecs[i];
+    const int64_t per = 256 * EPT;
+    int64_t chunks = (n + per - 1) / per;
+    chunks = std::max<int64_t>(1, std::min<int64_t>(chunks, (8192 + nb - 1) / nb));
+    hipLaunchKernelGGL((multi_lincomb_kernel<V, S, K, EPT>),
+                       dim3((unsigned)chunks, (unsigned)nb), dim3(256), 0, st, p, coef,
+                       (V*)out, n);
+}
+
+template <typename V, typename S>
+int lincomb_k(int k, int64_t nb, int64_t n, const void* const* vecs,
+              const double* coef, void* out, hipStream_t st) {
+    switch (k) {
+        case 1: launch_lincomb<V, S, 1>(nb, n, vecs, coef, out, st); break;
+        case 2: launch_lincomb<V, S, 2>(nb, n, vecs, coef, out, st); break;
+        case 3: launch_lincomb<V, S, 3>(nb, n, vecs, coef, out, st); break;
+        case 4: launch_lincomb<V, S, 4>(nb, n, vecs, coef, out, st); break;
+        case 5: launch_lincomb<V, S, 5>(nb, n, vecs, coef, out, st); break;
+        case 6: launch_lincomb<V, S, 6>(nb, n, vecs, coef, out, st); break;
+        case 7: launch_lincomb<V, S, 7>(nb, n, vecs, coef, out, st); break;
+        case 8: launch_lincomb<V, S, 8>(nb, n, vecs, coef, out, st); break;
+        default: set_error("multi_lincomb: k = %d out of range 1..%d", k, kLincombMax);
+                 return GH_ERR_ARG;
+    }
+    GH_CHECK_LAUNCH();
+    return GH_OK;
+}
+
 __device__ __forceinline__ double dotc_term(double a, double b) { return a * b; }
 __device__ __forceinline__ double dotc_term(float a, float b) {
     return (double)a * (double)b;
@@ -296,6 +393,28 @@ int gh_multi_dotc(int dtype, int64_t nbands, int64_t n, const void* a_dev,
                        (const double*)work_dev, scale, out_dev);
     GH_CHECK_LAUNCH();
     return GH_OK;
+}
+
+int gh_multi_lincomb(int dtype, int64_t nbands, int64_t n, int k,
+                     const void* const* vecs_dev, const double* coef_dev,
+                     void* out_dev, void* stream) {
+    GH_REQUIRE(vecs_dev && coef_dev && out_dev && n >= 0 && nbands >= 0, GH_ERR_ARG,
+               "multi_lincomb: bad argument");
+    GH_REQUIRE(k >= 1 && k <= kLincombMax, GH_ERR_ARG,
+               "multi_lincomb: k = %d out of range 1..%d", k, kLincombMax);
+    GH_REQUIRE(nbands < 65536, GH_ERR_ARG, "multi_lincomb: too many bands per call");
+    for (int a = 0; a < k; a++)
+        GH_REQUIRE(vecs_dev[a], GH_ERR_ARG, "multi_lincomb: null vector %d", a);
+    if (nbands == 0 || n == 0) return GH_OK;
+    hipStream_t st = (hipStream_t)stream;
+    switch (dtype) {
+        case GH_F64: return lincomb_k<double, double>(k, nbands, n, vecs_dev, coef_dev, out_dev, st);
+        case GH_C128: return lincomb_k<cplx<double>, double>(k, nbands, n, vecs_dev, coef_dev, out_dev, st);
+        case GH_F32: return lincomb_k<float, float>(k, nbands, n, vecs_dev, coef_dev, out_dev, st);
+        case GH_C64: return lincomb_k<cplx<float>, float>(k, nbands, n, vecs_dev, coef_dev, out_dev, st);
+    }
+    set_error("multi_lincomb: bad dtype");
+    return GH_ERR_ARG;
 }
 
 int64_t gh_multi_gram_work(int k, int64_t nbands) {

@@ -42,6 +42,9 @@ struct TGeom {
     int lx;        // output planes per block
     int nyt, nxc;
     int nry;       // input rows staged per tile (upper bound)
+    int zlo;       // ghost columns staged below z = 0
+    int zw;        // staged row width: zlo + n[2] + ghost columns above
+    int yz_trivial;  // Bloch phases along y and z are all 1 (k along x)
 };
 
 template <int K> struct W;
@@ -66,42 +69,97 @@ __device__ __forceinline__ bool wrap_idx(int& i, int n, int pbc, int ax,
     return true;
 }
 
+// (row, column) of the flat index threadIdx.x + t * blockDim.x in a
+// row-major array of width w, advanced with t without integer division.
+struct RowWalk {
+    int r, c, dr, dc, w;
+    __device__ __forceinline__ explicit RowWalk(int w_) : w(w_) {
+        r = threadIdx.x / w;
+        c = threadIdx.x - r * w;
+        dr = blockDim.x / w;
+        dc = blockDim.x - dr * w;
+    }
+    __device__ __forceinline__ void next() {
+        r += dr;
+        c += dc;
+        if (c >= w) {
+            c -= w;
+            r++;
+        }
+    }
+};
+
 // Register prefetch of a staged plane: thread t holds elements
 // t + s * blockDim.x (s < SPT) of the nrows x n2 stage, loaded while the
 // previous plane is filtered; the part of larger tiles beyond
 // SPT * blockDim.x is staged synchronously.
 template <typename V> struct StageRegs {
-    // real data only: for complex data the extra registers cost more than
-    // the prefetch gains (measured at 112^3 -> 56^3)
-    static constexpr int SPT = is_complex<V>::value ? 0 : 4;
-    V v[SPT > 0 ? SPT : 1];
+    static constexpr int SPT = 4;
+    // loads issued together by the synchronous part of the staging
+    static constexpr int kStageBatch = sizeof(V) <= 8 ? 8 : 4;
+    V v[SPT];
 };
 
+// Staged element i of plane ix (row i / zw, column i % zw, which is
+// z = column - zlo: each staged row carries its z ghosts): sets its offset
+// in the band and the Bloch phase of its periodic image; false for a zero
+// ghost (or, for the plane after the last one a block uses, beyond one
+// period: nothing is loaded).
+template <typename S>
+__device__ __forceinline__ bool stage_elem(int r, int col, int ix, int iy0,
+                                           const TGeom& g, const TPhase<S>& ph,
+                                           size_t& off, S& pre, S& pim) {
+    const int n2 = g.n[2];
+    int z = col - g.zlo;
+    int y = iy0 + r;
+    int x = ix;
+    pre = 1;
+    pim = 0;
+    if (!wrap_idx(x, g.n[0], g.pbc[0], 0, ph, pre, pim) || x < 0 || x >= g.n[0])
+        return false;
+    if (!wrap_idx(y, g.n[1], g.pbc[1], 1, ph, pre, pim)) return false;
+    if (!wrap_idx(z, n2, g.pbc[2], 2, ph, pre, pim) || z < 0 || z >= n2) return false;
+    off = ((size_t)x * g.n[1] + y) * n2 + z;
+    return true;
+}
+
+// A loaded element (zero for ghosts) with its Bloch phase applied; real
+// data need no phase.
+template <typename V, typename S>
+__device__ __forceinline__ V staged_value(V v, int r, int col, int ix, int iy0,
+                                          const TGeom& g, const TPhase<S>& ph) {
+    if constexpr (!is_complex<V>::value) {
+        return v;
+    } else {
+        if (g.yz_trivial) {
+            // only the x image carries a phase: one factor per plane
+            if (ix >= 0 && ix < g.n[0]) return v;
+            int x = ix;
+            S xre = 1, xim = 0;
+            wrap_idx(x, g.n[0], g.pbc[0], 0, ph, xre, xim);
+            return mul_phase(v, xre, xim);
+        }
+        size_t off;
+        S pre, pim;
+        stage_elem(r, col, ix, iy0, g, ph, off, pre, pim);
+        return mul_phase(v, pre, pim);
+    }
+}
+
+// Prefetch: raw values only, so that the loads stay in flight while the
+// previous plane is filtered; the phase is applied in stage_store.
 template <typename V, typename S>
 __device__ __forceinline__ void stage_load(const V* __restrict__ inb, int ix,
                                            int iy0, int nrows,
                                            StageRegs<V>& R, const TGeom& g,
-                                           const TPhase<S>& ph) {
-    const int n2 = g.n[2];
-    const int tot = nrows * n2;
-    S xre = 1, xim = 0;
-    int x = ix;
-    // the plane after the last one a block uses may lie beyond one period:
-    // nothing is loaded for it
-    bool xok = wrap_idx(x, g.n[0], g.pbc[0], 0, ph, xre, xim) && x >= 0 && x < g.n[0];
-    const V* pl = inb + (size_t)(xok ? x : 0) * g.n[1] * n2;
+                                           const TPhase<S>& ph, RowWalk w) {
 #pragma unroll
-    for (int s = 0; s < StageRegs<V>::SPT; s++) {
-        const int i = threadIdx.x + s * blockDim.x;
+    for (int s = 0; s < StageRegs<V>::SPT; s++, w.next()) {
         V v = vzero<V>();
-        if (i < tot && xok) {
-            int r = i / n2;
-            int z = i - r * n2;
-            int y = iy0 + r;
-            S pre = xre, pim = xim;
-            if (wrap_idx(y, g.n[1], g.pbc[1], 1, ph, pre, pim))
-                v = mul_phase(pl[y * n2 + z], pre, pim);
-        }
+        size_t off;
+        S pre, pim;
+        if (w.r < nrows && stage_elem(w.r, w.c, ix, iy0, g, ph, off, pre, pim))
+            v = inb[off];
         R.v[s] = v;
     }
 }
@@ -111,38 +169,37 @@ __device__ __forceinline__ void stage_store(const V* __restrict__ inb, int ix,
                                             int iy0, int nrows,
                                             const StageRegs<V>& R, V* A,
                                             const TGeom& g,
-                                            const TPhase<S>& ph) {
-    const int n2 = g.n[2];
-    const int tot = nrows * n2;
+                                            const TPhase<S>& ph, RowWalk w) {
 #pragma unroll
-    for (int s = 0; s < StageRegs<V>::SPT; s++) {
-        const int i = threadIdx.x + s * blockDim.x;
-        if (i < tot) A[i] = R.v[s];
-    }
-    const int done = StageRegs<V>::SPT * blockDim.x;
-    if (tot > done) {
-        S xre = 1, xim = 0;
-        int x = ix;
-        bool xok = wrap_idx(x, g.n[0], g.pbc[0], 0, ph, xre, xim);
-        const V* pl = inb + (size_t)x * g.n[1] * n2;
-        for (int i = done + threadIdx.x; i < tot; i += blockDim.x) {
-            int r = i / n2;
-            int z = i - r * n2;
-            int y = iy0 + r;
-            S pre = xre, pim = xim;
-            bool ok = xok && wrap_idx(y, g.n[1], g.pbc[1], 1, ph, pre, pim);
-            A[i] = ok ? mul_phase(pl[y * n2 + z], pre, pim) : vzero<V>();
+    for (int s = 0; s < StageRegs<V>::SPT; s++, w.next())
+        if (w.r < nrows)
+            A[w.r * g.zw + w.c] = staged_value(R.v[s], w.r, w.c, ix, iy0, g, ph);
+    // the rest: batches of kStageBatch loads in flight before their stores
+    constexpr int NB = StageRegs<V>::kStageBatch;
+    while (w.r < nrows) {
+        V v[NB];
+        RowWalk w0 = w;
+#pragma unroll
+        for (int s = 0; s < NB; s++, w.next()) {
+            size_t off;
+            S pre, pim;
+            v[s] = vzero<V>();
+            if (w.r < nrows && stage_elem(w.r, w.c, ix, iy0, g, ph, off, pre, pim))
+                v[s] = inb[off];
         }
+#pragma unroll
+        for (int s = 0; s < NB; s++, w0.next())
+            if (w0.r < nrows)
+                A[w0.r * g.zw + w0.c] = staged_value(v[s], w0.r, w0.c, ix, iy0, g, ph);
     }
 }
 
-// Value of staged row r at interior z index (wrapping along z).
+// Value of a staged row at z (row points at z = 0; the staged ghost
+// columns already carry their phase or zero).
 template <typename V, typename S>
-__device__ __forceinline__ V row_at(const V* row, int z, const TGeom& g,
-                                    const TPhase<S>& ph) {
-    S pre = 1, pim = 0;
-    if (!wrap_idx(z, g.n[2], g.pbc[2], 2, ph, pre, pim)) return vzero<V>();
-    return mul_phase(row[z], pre, pim);
+__device__ __forceinline__ V row_at(const V* row, int z, const TGeom&,
+                                    const TPhase<S>&) {
+    return row[z];
 }
 
 template <typename V, typename S, int K>
@@ -166,8 +223,9 @@ restrict_kernel(const V* __restrict__ in, V* __restrict__ out, TGeom g,
                 TPhase<S> ph, S alpha) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int n2 = g.n[2], m2 = g.m[2];
-    V* A = reinterpret_cast<V*>(smem_raw);   // nry x n2
-    V* B = A + g.nry * n2;                   // nry x m2
+    V* A = reinterpret_cast<V*>(smem_raw);   // nry x zw
+    V* B = A + g.nry * g.zw;                 // nry x m2
+    const RowWalk wz(g.zw), wm(m2);          // thread's staged / filtered element
     const int per_band = g.nyt * g.nxc;
     const int b = xcd_swizzle(blockIdx.x, gridDim.x);
     const int band = b / per_band;
@@ -203,15 +261,15 @@ restrict_kernel(const V* __restrict__ in, V* __restrict__ out, TGeom g,
     // plane's filters); the next plane ix + 1 is loaded during this one's
     StageRegs<V> pre;
     auto process_plane = [&](int ix, int slot) {
-        stage_store<V, S>(inb, ix, iy0, nrows, pre, A, g, ph);
+        stage_store<V, S>(inb, ix, iy0, nrows, pre, A, g, ph, wz);
         __syncthreads();
-        stage_load<V, S>(inb, ix + 1, iy0, nrows, pre, g, ph);
+        stage_load<V, S>(inb, ix + 1, iy0, nrows, pre, g, ph, wz);
         // z-filter every staged row
-        for (int i = threadIdx.x; i < nrows * m2; i += blockDim.x) {
-            int r = i / m2;
-            int kk = i - r * m2;
+        RowWalk w = wm;
+        for (; w.r < nrows; w.next()) {
+            const int r = w.r, kk = w.c;
             int c = 2 * kk + (K - 1) - g.plo[2];
-            B[r * m2 + kk] = restrict_tap<V, S, K>(A + r * n2, c, g, ph);
+            B[r * m2 + kk] = restrict_tap<V, S, K>(A + r * g.zw + g.zlo, c, g, ph);
         }
         __syncthreads();
         // y-filter (rows are already ghost-filled)
@@ -232,7 +290,7 @@ restrict_kernel(const V* __restrict__ in, V* __restrict__ out, TGeom g,
     };
 
     // prime the first 2K-3 planes
-    stage_load<V, S>(inb, base, iy0, nrows, pre, g, ph);
+    stage_load<V, S>(inb, base, iy0, nrows, pre, g, ph, wz);
 #pragma unroll
     for (int s = 0; s < QL - 2; s++) process_plane(base + s, s);
     for (int I = I0; I < I1; I++) {
@@ -283,8 +341,9 @@ interpolate_kernel(const V* __restrict__ in, V* __restrict__ out, TGeom g,
     extern __shared__ __align__(16) unsigned char smem_raw[];
     constexpr int nn = K / 2;
     const int n2 = g.n[2], m2 = g.m[2];
-    V* A = reinterpret_cast<V*>(smem_raw);   // nry x n2
-    V* B = A + g.nry * n2;                   // nry x m2
+    V* A = reinterpret_cast<V*>(smem_raw);   // nry x zw
+    V* B = A + g.nry * g.zw;                 // nry x m2
+    const RowWalk wz(g.zw), wm(m2);          // thread's staged / filtered element
     const int per_band = g.nyt * g.nxc;
     const int b = xcd_swizzle(blockIdx.x, gridDim.x);
     const int band = b / per_band;
@@ -318,13 +377,13 @@ interpolate_kernel(const V* __restrict__ in, V* __restrict__ out, TGeom g,
     // plane ix is in registers on entry; ix + 1 is loaded during its filters
     StageRegs<V> pre;
     auto process_plane = [&](int ix, int slot) {
-        stage_store<V, S>(inb, ix, cy0, nrows, pre, A, g, ph);
+        stage_store<V, S>(inb, ix, cy0, nrows, pre, A, g, ph, wz);
         __syncthreads();
-        stage_load<V, S>(inb, ix + 1, cy0, nrows, pre, g, ph);
-        for (int i = threadIdx.x; i < nrows * m2; i += blockDim.x) {
-            int r = i / m2;
-            int kk = i - r * m2;
-            B[r * m2 + kk] = interp_tap<V, S, K>(A + r * n2, kk, g.skip[2],
+        stage_load<V, S>(inb, ix + 1, cy0, nrows, pre, g, ph, wz);
+        RowWalk w = wm;
+        for (; w.r < nrows; w.next()) {
+            const int r = w.r, kk = w.c;
+            B[r * m2 + kk] = interp_tap<V, S, K>(A + r * g.zw + g.zlo, kk, g.skip[2],
                                                  g.plo[2], g, ph);
         }
         __syncthreads();
@@ -354,7 +413,7 @@ interpolate_kernel(const V* __restrict__ in, V* __restrict__ out, TGeom g,
     const int i_first = (O0 + g.skip[0]) >> 1;
     const int i_last = (O1 - 1 + g.skip[0]) >> 1;
     const int cbase = nn - 1 - g.plo[0];
-    stage_load<V, S>(inb, i_first + cbase - nn + 1, cy0, nrows, pre, g, ph);
+    stage_load<V, S>(inb, i_first + cbase - nn + 1, cy0, nrows, pre, g, ph, wz);
 #pragma unroll
     for (int s = 0; s < K - 1; s++) process_plane(i_first + cbase - nn + 1 + s, s);
     for (int i = i_first; i <= i_last; i++) {
@@ -412,20 +471,30 @@ TPhase<S> tphase(const double* phase_cd) {
     return a;
 }
 
-constexpr int kPPT = 2;
+// Output points per thread: two, except for restriction with the long
+// filter or complex128 data, where the register queue of 2K - 1 filtered
+// planes per point would otherwise limit occupancy (measured at
+// 112^3 -> 56^3).
+template <typename V, int K>
+constexpr int ppt_of(bool interp) {
+    return (!interp && (K >= 6 || sizeof(V) > 8)) ? 1 : 2;
+}
 
 // Rows of the output tile: as many as fit 1024 threads and 64 KB LDS.
-int choose_rows(const gh_transformer* h, int vsize, size_t* lds_out,
+int choose_rows(const gh_transformer* h, int vsize, int ppt, size_t* lds_out,
                 int* nry_out) {
     const TGeom& g = h->g;
     const int K = 2 * h->nn;
     int best = 0;
     size_t best_lds = 0;
     int best_nry = 0;
-    for (int tj = 1; tj <= g.m[1]; tj++) {
+    // the 2-tap restriction (V-cycle) is fastest with short tiles and more
+    // blocks (measured at 112^3 -> 56^3, 256 bands: 4-17 % over 14 rows)
+    const int maxrows = (!h->interp && h->nn == 1) ? 7 : g.m[1];
+    for (int tj = 1; tj <= std::min(g.m[1], maxrows); tj++) {
         int nry = h->interp ? (tj + 1) / 2 + 2 * h->nn + 1 : 2 * tj + 2 * K - 3;
-        size_t lds = (size_t)nry * (g.n[2] + g.m[2]) * vsize;
-        int thr = (tj * g.m[2] + kPPT - 1) / kPPT;
+        size_t lds = (size_t)nry * (g.zw + g.m[2]) * vsize;
+        int thr = (tj * g.m[2] + ppt - 1) / ppt;
         if (lds > 64 * 1024 || thr > 1024) break;
         if (g.m[1] % tj == 0 || tj == g.m[1] || best == 0) {
             best = tj;
@@ -445,7 +514,10 @@ int launch_t(const gh_transformer* h, const void* in, void* out,
     TGeom g = h->g;
     size_t lds;
     int nry;
-    int tj = choose_rows(h, sizeof(V), &lds, &nry);
+    constexpr int PPT_R = ppt_of<V, K>(false);
+    constexpr int PPT_I = ppt_of<V, K>(true);
+    const int ppt = h->interp ? PPT_I : PPT_R;
+    int tj = choose_rows(h, sizeof(V), ppt, &lds, &nry);
     GH_REQUIRE(tj > 0, GH_ERR_UNSUPPORTED,
                "transformer: grid rows too long for one block");
     g.tj = tj;
@@ -458,16 +530,20 @@ int launch_t(const gh_transformer* h, const void* in, void* out,
         g.lx = (g.lx + 1) / 2;
         g.nxc = (g.m[0] + g.lx - 1) / g.lx;
     }
-    int thr = (tj * g.m[2] + kPPT - 1) / kPPT;
+    int thr = (tj * g.m[2] + ppt - 1) / ppt;
     thr = ((thr + 63) / 64) * 64;
     int64_t nblocks = nbands * (int64_t)g.nyt * g.nxc;
     GH_REQUIRE(nblocks < (1ll << 31), GH_ERR_SHAPE, "transformer: too many blocks");
     TPhase<S> ph = tphase<S>(phase_cd);
+    g.yz_trivial = 1;
+    for (int c = 1; c < 3; c++)
+        for (int d = 0; d < 2; d++)
+            if (ph.re[c][d] != (S)1 || ph.im[c][d] != (S)0) g.yz_trivial = 0;
     if (h->interp)
-        hipLaunchKernelGGL((interpolate_kernel<V, S, K, kPPT>), dim3((unsigned)nblocks),
+        hipLaunchKernelGGL((interpolate_kernel<V, S, K, PPT_I>), dim3((unsigned)nblocks),
                            dim3(thr), lds, st, (const V*)in, (V*)out, g, ph, (S)alpha);
     else
-        hipLaunchKernelGGL((restrict_kernel<V, S, K, kPPT>), dim3((unsigned)nblocks),
+        hipLaunchKernelGGL((restrict_kernel<V, S, K, PPT_R>), dim3((unsigned)nblocks),
                            dim3(thr), lds, st, (const V*)in, (V*)out, g, ph, (S)alpha);
     GH_CHECK_LAUNCH();
     return GH_OK;
@@ -522,6 +598,27 @@ int gh_transformer_create(gh_transformer** out, const int64_t* nin_c,
             set_error("transformer_create: inconsistent sizes/pads on axis %d", c);
             return GH_ERR_SHAPE;
         }
+    }
+    // z ghost columns staged with each row: the filter taps span
+    // [zmin, zmax] in interior z coordinates
+    {
+        const int n2 = h->g.n[2], m2 = h->g.m[2], p2 = h->g.plo[2], s2 = h->g.skip[2];
+        int zmin, zmax;
+        if (h->interp) {
+            zmin = (s2 >> 1) - p2;
+            zmax = ((m2 - 1 + s2) >> 1) + 2 * nn - 1 - p2;
+        } else {
+            zmin = -p2;
+            zmax = 2 * (m2 - 1) + 2 * (2 * nn - 1) - p2;
+        }
+        const int zlo = std::max(0, -zmin), zhi = std::max(0, zmax - (n2 - 1));
+        if (h->g.pbc[2] && (zlo > n2 || zhi > n2)) {
+            delete h;
+            set_error("transformer_create: z ghosts wider than the grid");
+            return GH_ERR_SHAPE;
+        }
+        h->g.zlo = zlo;
+        h->g.zw = zlo + n2 + zhi;
     }
     *out = h;
     return GH_OK;

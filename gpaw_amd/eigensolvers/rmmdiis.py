@@ -24,10 +24,10 @@ Grid work (stencils, projectors, V-cycle, band-wise dots and axpys) runs in
 libgpawhip; the nbands x nbands algebra (S, H, eigh, Cholesky, rotations)
 runs on the GPU in FP64 through torch (rocBLAS / rocSOLVER).
 """
-import numpy as np
 import torch
 
 from .. import blas
+from .eigensolver import Eigensolver, _Kpt, band_weights
 
 
 class FDWaveFunctions:
@@ -86,12 +86,7 @@ class FDWaveFunctions:
         self.orthonormalized = True
 
 
-class _Kpt:
-    def __init__(self, phase_cd):
-        self.phase_cd = phase_cd
-
-
-class RMMDIIS:
+class RMMDIIS(Eigensolver):
     """RMM-DIIS eigensolver (rmmdiis.py:10-50 parameters).
 
     blocksize: bands updated together.  Every step of the iteration is per
@@ -114,45 +109,6 @@ class RMMDIIS:
         self.error = None
 
     # ----------------------------------------------------------- pieces --
-    def apply_h_local(self, ham, wfs, a, out):
-        ham.apply_local(a, out, wfs.phase_cd)
-
-    def calculate_residuals(self, ham, wfs, psit, P, eps, R):
-        """R <- R - eps psit + sum_a p_a (P dH_a - eps P dS_a), where R holds
-        (T + v) psit on entry (eigensolver.py:153-172)."""
-        blas.multi_axpy(-eps, psit, R)
-        if ham.pt is not None:
-            ham.pt.add_M(R, ham.dH(P, eps_x=eps))
-
-    def subspace_diagonalize(self, ham, wfs):
-        """Rotate psit, P to the eigenvectors of the subspace Hamiltonian;
-        returns (T + v) psit of the rotated wave functions."""
-        psit = wfs.psit
-        Htpsit = torch.empty_like(psit)
-        self.apply_h_local(ham, wfs, psit, Htpsit)
-        psi = wfs.flat(psit)
-        hpsi = wfs.flat(Htpsit)
-        H = wfs.dv * (psi.conj() @ hpsi.T)
-        P = wfs.projections()
-        if P is not None:
-            H = H + P.conj() @ ham.dH(P).T
-        H = 0.5 * (H + H.conj().T)
-        eps, C = torch.linalg.eigh(H)
-        U = C.T.contiguous()
-        wfs.psit = (U @ psi).reshape(psit.shape)
-        Htpsit = (U @ hpsi).reshape(psit.shape)
-        if P is not None:
-            wfs.P = U @ P
-        wfs.eps = eps.to(torch.float64).contiguous()
-        return Htpsit
-
-    def integrate(self, wfs, a, b):
-        """Re <a_x|b_x> per band."""
-        return blas.multi_dotc(a, b, scale=wfs.dv)
-
-    def precondition(self, wfs, R, kpt):
-        return wfs.preconditioner(R, kpt)
-
     def residual_of(self, ham, wfs, x, eps, out):
         """out = (H - eps S) x."""
         self.apply_h_local(ham, wfs, x, out)
@@ -176,54 +132,67 @@ class RMMDIIS:
 
     def iterate_one_k_point(self, ham, wfs, weights=None):
         nb = wfs.nbands
-        if weights is None:
-            weights = torch.ones(nb, dtype=torch.float64, device='cuda')
-        else:
-            weights = torch.as_tensor(weights, dtype=torch.float64,
-                                      device='cuda')
+        weights = band_weights(weights, nb)
         R = self.subspace_diagonalize(ham, wfs)
         psit = wfs.psit
         self.calculate_residuals(ham, wfs, psit, wfs.P, wfs.eps, R)
         kpt = _Kpt(wfs.phase_cd)
         B = self.blocksize or nb
         errors = []
-        dR_full = None
         for n1 in range(0, nb, B):
             n2 = min(n1 + B, nb)
             psib = psit[n1:n2]
             Rb = R[n1:n2]
             eps = wfs.eps[n1:n2].contiguous()
-            errors.append(weights[n1:n2] * self.integrate(wfs, Rb, Rb))
-            if self.niter > 1:
-                psi_hist = [psib.clone()]
-                R_hist = [Rb.clone()]
-            dpsit = self.precondition(wfs, Rb, kpt)
-            if dR_full is None:
-                dR_full = torch.empty_like(dpsit)
-            dR = dR_full[:n2 - n1]
-            self.residual_of(ham, wfs, dpsit, eps, dR)
-            RdR = self.integrate(wfs, dR, Rb)
-            dRdR = self.integrate(wfs, dR, dR)
-            lam = -RdR / dRdR
-            if self.limit_lambda:
-                lam = self._limit(lam)
-            blas.multi_axpy(lam, dpsit, psib)
-            blas.multi_axpy(lam, dR, Rb)
-            for nit in range(1, self.niter):
-                psi_hist.append(psib.clone())
-                R_hist.append(Rb.clone())
-                if nit > 1 or self.limit_lambda:
-                    self._diis_combine(wfs, psi_hist, R_hist, psib, Rb, nit)
-                if nit < self.niter - 1:
-                    dpsit = self.precondition(wfs, Rb, kpt)
-                    blas.multi_axpy(lam, dpsit, psib)
-                    self.residual_of(ham, wfs, psib, eps, Rb)
-            dpsit = self.precondition(wfs, Rb, kpt)
-            if self.trial_step is not None:
-                lam = torch.full((n2 - n1,), self.trial_step,
-                                 dtype=torch.float64, device='cuda')
-            blas.multi_axpy(lam, dpsit, psib)
+            errors.append(weights[n1:n2] * self._update_block(
+                ham, wfs, kpt, psib, Rb, eps))
         return float(torch.cat(errors).sum())
+
+    def _step(self, x, lam, dx, out):
+        """out = x + lam dx per band, one pass (out may be dx)."""
+        one = torch.ones_like(lam)
+        return blas.multi_lincomb([x, dx], torch.stack([one, lam], 1), out)
+
+    def _update_block(self, ham, wfs, kpt, psib, Rb, eps):
+        """The DIIS iterations of one block (rmmdiis.py:124-254); psib is
+        overwritten with the result, Rb is used as workspace.  Returns the
+        per-band errors dv <R|R> of the residuals on entry.
+
+        The reference copies each iterate into its DIIS history and updates
+        the block in place; here each iterate is a buffer of its own that the
+        history keeps, and every update x + lam dx or sum_i alpha_i x_i is one
+        pass that writes a new iterate (no copies)."""
+        # RdR = dv <dR|R>, dRdR = dv <dR|dR> and the error dv <R|R> from one
+        # pass over dR and R
+        dpsit = self.precondition(wfs, Rb, kpt)
+        dR = torch.empty_like(dpsit)
+        self.residual_of(ham, wfs, dpsit, eps, dR)
+        G = blas.multi_gram([dR, Rb], scale=wfs.dv)
+        if G.is_complex():
+            G = G.real
+        error = G[:, 1, 1]
+        lam = -G[:, 0, 1] / G[:, 0, 0]
+        if self.limit_lambda:
+            lam = self._limit(lam)
+        psi_hist = [psib, self._step(psib, lam, dpsit, dpsit)]
+        R_hist = [Rb, self._step(Rb, lam, dR, dR)]
+        psi_c, R_c = psi_hist[-1], R_hist[-1]
+        for nit in range(1, self.niter):
+            if nit > 1 or self.limit_lambda:
+                psi_c, R_c = self._diis_combine(wfs, psi_hist, R_hist, nit)
+            if nit < self.niter - 1:
+                dpsit = self.precondition(wfs, R_c, kpt)
+                psi_n = self._step(psi_c, lam, dpsit, dpsit)
+                R_n = torch.empty_like(psi_n)
+                self.residual_of(ham, wfs, psi_n, eps, R_n)
+                psi_hist.append(psi_n)
+                R_hist.append(R_n)
+                psi_c, R_c = psi_n, R_n
+        dpsit = self.precondition(wfs, R_c, kpt)
+        if self.trial_step is not None:
+            lam = torch.full_like(lam, self.trial_step)
+        self._step(psi_c, lam, dpsit, psib)
+        return error
 
     def _limit(self, lam):
         lower = self.limit_lambda['lower']
@@ -235,11 +204,12 @@ class RMMDIIS:
             lam = lam.clamp(lower, upper)
         return lam
 
-    def _diis_combine(self, wfs, psi_hist, R_hist, psib, Rb, nit):
+    def _diis_combine(self, wfs, psi_hist, R_hist, nit):
         """Per band: alpha minimising |sum_i alpha_i R_i| with
-        sum_i alpha_i = 1 over the nit + 1 stored iterates, then
-        psi = sum_i alpha_i psi_i, R = sum_i alpha_i R_i
-        (rmmdiis.py:200-239)."""
+        sum_i alpha_i = 1 over the nit + 1 stored iterates; returns
+        (sum_i alpha_i psi_i, sum_i alpha_i R_i) as new arrays, the terms
+        added in the reference's order: alpha_nit x_nit first, then
+        i = 0 .. nit - 1 (rmmdiis.py:200-239)."""
         k = nit + 1
         # R_nn[b, i, j] = dv <R_i|R_j> (one pass over the k residuals)
         R_nn = blas.multi_gram(R_hist[:k], scale=wfs.dv)
@@ -250,8 +220,7 @@ class RMMDIIS:
         x = torch.zeros((nbk, k + 1, 1), dtype=R_nn.dtype, device='cuda')
         x[:, -1, 0] = -1.0
         alpha = torch.linalg.solve(A, x)[:, :k, 0]
-        shape = (nbk,) + (1,) * (psib.dim() - 1)
-        for hist, out in ((psi_hist, psib), (R_hist, Rb)):
-            torch.mul(hist[nit], alpha[:, nit].reshape(shape), out=out)
-            for i in range(nit):
-                out.addcmul_(hist[i], alpha[:, i].reshape(shape))
+        order = [nit] + list(range(nit))
+        coef = alpha[:, order].contiguous()
+        return tuple(blas.multi_lincomb([hist[i] for i in order], coef)
+                     for hist in (psi_hist, R_hist))

@@ -215,17 +215,30 @@ int gh_multi_dotc(int dtype, int64_t nbands, int64_t n, const void* a_dev,
  * (nbands, k, k) doubles for real dtypes, (nbands, k, k, 2) (re, im) for
  * complex ones (Hermitian).  work_dev: gh_multi_gram_work(k, nbands)
  * doubles.  FP64 accumulation in a fixed order. */
+int64_t gh_multi_gram_work(int k, int64_t nbands);
+int gh_multi_gram(int dtype, int64_t nbands, int64_t n, int k,
+                  const void* const* vecs_dev, double scale, double* out_dev,
+                  double* work_dev, void* stream);
+
+/* Per-band linear combination of k vectors, one pass over the data:
+ *   out[x, G] = sum_{i<k} c[x, i] v_i[x, G],  1 <= k <= 8,
+ * accumulated in the order given.  Replaces the DIIS update of
+ * rmmdiis.py:226-237 (psitb = alpha_nit psi_nit, then axpy of the older
+ * iterates) and the out-of-place form of the per-band axpy of
+ * mblas.py:13-32 (v_0 + lam v_1).  vecs_dev: host array of k device
+ * pointers (nbands, n) of `dtype`; coef_dev: device (nbands, k) doubles for
+ * real dtypes, (nbands, k, 2) (re, im) for complex ones.  out_dev may be
+ * one of the inputs. */
+int gh_multi_lincomb(int dtype, int64_t nbands, int64_t n, int k,
+                     const void* const* vecs_dev, const double* coef_dev,
+                     void* out_dev, void* stream);
+
 /* Pseudo-density accumulation, replaces the per-band calls of
  * cgpaw.add_to_density (c/utilities.c:313-356; wavefunctions/fd.py:95-100):
  *   nt[G] += sum_{x < nbands} f[x] |psit[x, G]|^2   (FP64 nt, any dtype)
  * f_dev: nbands doubles on the device; bands are added in order. */
 int gh_add_to_density(int dtype, int64_t nbands, int64_t n, const double* f_dev,
                       const void* psit_dev, double* nt_dev, void* stream);
-
-int64_t gh_multi_gram_work(int k, int64_t nbands);
-int gh_multi_gram(int dtype, int64_t nbands, int64_t n, int k,
-                  const void* const* vecs_dev, double scale, double* out_dev,
-                  double* work_dev, void* stream);
 
 #ifdef __cplusplus
 }

@@ -21,6 +21,7 @@ precond     the 14-step multigrid V-cycle (preconditioner.py:35-93)
 rmmdiis     Hamiltonian with dH/dS, subspace diagonalisation, inverse-Cholesky
             orthonormalisation and the blocked RMM-DIIS sweep
             (eigensolver.py:153-236, fdpw.py:394-424, rmmdiis.py:63-254)
+davidson    one Davidson iterate (eigensolvers/davidson.py:80-222)
 poisson     Jacobi relaxation and the multigrid Poisson solver
             (c/bmgs/relax.c, c/operators.c:55-95, poisson.py:318-546)
 density     pseudo-density accumulation (c/utilities.c:313-356)

@@ -154,3 +154,47 @@ def test_blocked_vs_unblocked():
     assert np.allclose(e1, e2, rtol=1e-12, atol=0)
     assert abs(eps1 - eps2).max() < 1e-12
     assert abs(p1 - p2).max() < 1e-11 * abs(p2).max()
+
+
+@pytest.mark.parametrize('k_c', CASES)
+def test_davidson_iterate_vs_oracle(k_c):
+    """One Davidson iterate (two expansions) from the same start: errors,
+    eigenvalues and wave functions (up to a phase per band)."""
+    from gpaw_amd.eigensolvers import Davidson
+    from oracle import davidson as odv
+    ham_o, pc_o, rng = small_system(N=12, h=0.4, k_c=k_c, seed=9)
+    gd, ham, pc = gpu_system(ham_o)
+    Hm, Sm = dense_pair(ham_o)
+    nb = 6
+    w, W = sl.eigh(Hm, Sm, subset_by_index=[0, nb - 1])
+    psi, _ = orm.orthonormalize(ham_o, start_guess(ham_o, rng, nb, W, 0.3))
+    err_o, eps_o, psi_o = odv.davidson_iterate(ham_o, psi.copy(), pc_o)
+    wfs = FDWaveFunctions(gd, dev(psi), ham.pt, pc, ham_o.phase_cd)
+    wfs.orthonormalized = True
+    err = Davidson().iterate(ham, wfs)
+    assert abs(err - err_o) < 1e-10 * err_o
+    assert abs(host(wfs.eps) - eps_o).max() < 1e-11 * abs(eps_o).max()
+    out = host(wfs.psit)
+    assert abs(align(out, psi_o) - psi_o).max() < 1e-9 * abs(psi_o).max()
+    # the projections were rotated with the wave functions
+    P_new = host(ham.pt.integrate_M(wfs.psit))
+    assert abs(host(wfs.P) - P_new).max() < 1e-11 * abs(P_new).max()
+
+
+@pytest.mark.parametrize('k_c', CASES)
+def test_davidson_converges_to_dense(k_c):
+    """Eight bands, the lowest six converged (zero weights on two buffer
+    bands, as the reference's nbands_converge does)."""
+    from gpaw_amd.eigensolvers import Davidson
+    ham_o, _, rng = small_system(N=12, h=0.4, k_c=k_c, seed=10)
+    gd, ham, pc = gpu_system(ham_o)
+    Hm, Sm = dense_pair(ham_o)
+    nb, nconv = 8, 6
+    w, W = sl.eigh(Hm, Sm, subset_by_index=[0, nb - 1])
+    psi = start_guess(ham_o, rng, nb, W, 0.3)
+    wfs = FDWaveFunctions(gd, dev(psi), ham.pt, pc, ham_o.phase_cd)
+    weights = [1.0] * nconv + [0.0] * (nb - nconv)
+    es = Davidson()
+    errors = [es.iterate(ham, wfs, weights) for _ in range(30)]
+    assert errors[-1] < 1e-8 * errors[0]
+    assert abs(host(wfs.eps)[:nconv] - w[:nconv]).max() < 1e-9

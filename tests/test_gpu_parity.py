@@ -189,7 +189,7 @@ def test_stencil_rejects_aliasing():
 TRANSFORMER_CASES = [(nn, dt, pbc, k) for nn in (1, 2, 3, 4)
                      for dt in (np.float64, np.complex128)
                      for pbc in (True, (True, False, True))
-                     for k in ((0.25, -0.1, 0.3),)]
+                     for k in ((0.25, -0.1, 0.3), (0.25, 0, 0))]
 
 
 @pytest.mark.parametrize('nn,dtype,pbc,k_c', TRANSFORMER_CASES)
@@ -405,6 +405,37 @@ def test_multi_gram(dtype, k):
                           np.array([v.reshape(5, -1) for v in vs]).conj(),
                           np.array([v.reshape(5, -1) for v in vs]))
     np.testing.assert_allclose(G, ref, rtol=1e-13, atol=1e-13 * abs(ref).max())
+
+
+@pytest.mark.parametrize('dtype', [np.float64, np.complex128, np.float32,
+                                   np.complex64])
+@pytest.mark.parametrize('k', [1, 2, 3, 8])
+def test_multi_lincomb(dtype, k):
+    """Per-band linear combinations (the DIIS update and x + lam dx)
+    against numpy, terms added in the order given (the first term is a
+    product, the rest are FMAs, so the error is a few ulps of the largest
+    term); also written over one of its inputs."""
+    from gpaw_amd import blas
+    rng = np.random.default_rng(15)
+    nb, shape = 5, (5, 7, 9, 11)
+    vs = [rand(rng, shape, dtype) for _ in range(k)]
+    cplx = np.iscomplexobj(vs[0])
+    c = rng.standard_normal((nb, k))
+    if cplx:
+        c = c + 1j * rng.standard_normal((nb, k))
+    ref = sum(c[:, i].reshape(-1, 1, 1, 1) * vs[i].astype(np.complex128)
+              for i in range(k))
+    if not cplx:
+        ref = ref.real
+    tol = 1e-6 if dtype in (np.float32, np.complex64) else 1e-14
+    d = [dev(v) for v in vs]
+    out = blas.multi_lincomb(d, torch.from_numpy(c).cuda())
+    assert relerr(host(out), ref) < tol
+    blas.multi_lincomb(d, torch.from_numpy(c).cuda(), out=d[-1])
+    assert relerr(host(d[-1]), ref) < tol
+    if not cplx:
+        with pytest.raises(AssertionError):
+            blas.multi_lincomb(d, torch.from_numpy(c + 1j).cuda())
 
 
 @pytest.mark.parametrize('dtype', [np.float64, np.complex128, np.float32,
