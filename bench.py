@@ -55,6 +55,8 @@ def parse():
                    help='Davidson iterates (0: skip the Davidson leg)')
     p.add_argument('--rmm-converge', type=int, default=64,
                    help='lowest bands whose residuals make up the error')
+    p.add_argument('--no-scf-next', action='store_true',
+                   help='skip the pseudo-density / Poisson leg')
     p.add_argument('--no-transformers', action='store_true',
                    help='skip the restrict / interpolate leg')
     p.add_argument('--no-c3', action='store_true',
@@ -302,6 +304,58 @@ def transformer_leg(gd, nb):
     return out
 
 
+def scf_next_leg(w, gd, psi):
+    """The steps after the eigensolver in an SCF iteration, on the same
+    workload: the pseudo-density nt = sum_n f_n |psi_n|^2 over the 256
+    bands (FP64), and the multigrid Poisson solver (nn = 3, Jacobi) on the
+    fine grid (2N)^3 for a neutral charge: one Gaussian per atom
+    (exp(-r^2 / (1 Bohr)^2), minimum image) minus its mean."""
+    import torch
+    from gpaw_amd.density import add_to_density_bands
+    from gpaw_amd.grid_descriptor import GridDescriptor
+    from gpaw_amd.poisson import FDPoissonSolver
+    nb = psi.shape[0]
+    N = int(np.prod(gd.n_c))
+    res = {}
+    nt = torch.zeros(tuple(gd.n_c), dtype=torch.float64, device='cuda')
+    f = torch.full((nb,), 2.0, dtype=torch.float64, device='cuda')
+    t = _time_ms(lambda: add_to_density_bands(f, psi, nt))
+    res['density_ms'] = round(t, 4)
+    res['density_GBps'] = round((nb + 2) * N * 8 / t / 1e6, 1)
+    del nt
+    nf = 2 * int(gd.n_c[0])
+    gdf = GridDescriptor((nf,) * 3, (w.L,) * 3)
+    h = w.L / nf
+    ax = torch.arange(nf, dtype=torch.float64, device='cuda') * h
+    rho = torch.zeros((nf,) * 3, dtype=torch.float64, device='cuda')
+    for p in w.pos_av:
+        d = []
+        for c in range(3):
+            x = ax - float(p[c])
+            d.append(x - w.L * torch.round(x / w.L))
+        rho += torch.exp(-(d[0][:, None, None]**2 + d[1][None, :, None]**2 +
+                           d[2][None, None, :]**2))
+    rho -= rho.mean()
+    ps = FDPoissonSolver(nn=3)
+    ps.set_grid_descriptor(gdf)
+    phi = torch.zeros_like(rho)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    niter = ps.solve_neutral(phi, rho)
+    torch.cuda.synchronize()
+    t_solve = time.perf_counter() - t0
+    res['poisson_grid'] = [nf] * 3
+    res['poisson_levels'] = ps.levels
+    res['poisson_vcycles_to_2e-10'] = niter
+    res['poisson_solve_ms'] = round(1e3 * t_solve, 2)
+    res['poisson_vcycle_ms'] = round(_time_ms(lambda: ps.iterate2(ps.step)), 3)
+    op = ps.operators[0]
+    t = _time_ms(lambda: op.relax(2, phi, rho, 1, 2.0 / 3.0))
+    res['jacobi_sweep_ms'] = round(t, 4)
+    res['jacobi_sweep_GBps'] = round(3 * nf**3 * 8 / t / 1e6, 1)
+    return res
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -359,6 +413,10 @@ def main():
         out = None
         trans = transformer_leg(gd, nb)
 
+    scf_next = None
+    if not args.no_scf_next:
+        scf_next = scf_next_leg(w, gd, psi)
+
     rmm = None
     if args.rmm_sweeps > 0:
         out = None
@@ -410,6 +468,7 @@ def main():
         'davidson': dav,
         'c3_single_precision': c3,
         'transformers': trans,
+        'density_and_poisson': scf_next,
         'cpu_baseline': cpu,
     }
     print(json.dumps(line))
