@@ -31,6 +31,7 @@ _SIGNATURES = {
     'gh_stencil_destroy': (_c.c_int, [P]),
     'gh_stencil_diagonal': (D, [P]),
     'gh_stencil_apply': (_c.c_int, [P, P, P, I64, PD, P, P, D, P]),
+    'gh_stencil_apply_shifted': (_c.c_int, [P, P, P, I64, PD, P, P, P]),
     'gh_stencil_set_tile': (_c.c_int, [P, _c.c_int, _c.c_int, _c.c_int]),
     'gh_stencil_relax': (_c.c_int, [P, _c.c_int, P, P, P, _c.c_int, D, P]),
     'gh_transformer_create': (_c.c_int, [_c.POINTER(P), PI64, PI64, _c.c_int,

@@ -11,6 +11,15 @@ int stencil_apply_scaled(const gh_stencil* h, const void* in, void* out,
                          const void* vt, const void* src, double beta,
                          double alpha, double gamma, hipStream_t st);
 
+// as stencil_apply_scaled, with the centre coefficient alpha c0 + gamma -
+// eps[band] per band (eps: nbands device doubles or null); orthogonal
+// stencils only
+int stencil_apply_full(const gh_stencil* h, const void* in, void* out,
+                       int64_t nbands, const double* phase_cd,
+                       const void* vt, const void* src, double beta,
+                       double alpha, double gamma, const double* eps,
+                       hipStream_t st);
+
 // out = alpha * T(in)
 int transformer_apply_scaled(const gh_transformer* h, const void* in,
                              void* out, int64_t nbands,

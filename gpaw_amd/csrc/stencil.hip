@@ -47,6 +47,7 @@ constexpr int kMaxGeneric = 96; // stencil points of the generic kernel
 template <typename S>
 struct CfdCoefs {
     S c0;
+    const double* eps;  // per band: centre coefficient c0 - eps[band] (or null)
     S cm[3][kMaxR];  // coefficient of offset -(j+1) along axis c
     S cp[3][kMaxR];  // coefficient of offset +(j+1) along axis c
 };
@@ -96,6 +97,7 @@ stencil_cfd_kernel(const V* __restrict__ in, V* __restrict__ out,
     const int b = xcd_swizzle(blockIdx.x, gridDim.x);
     const int band = b / tiles_per_band;
     const int rem = b - band * tiles_per_band;
+    const S c0b = cf.eps ? cf.c0 - (S)cf.eps[band] : cf.c0;
     const int xc = rem / g.nyt;
     const int yt = rem - xc * g.nyt;
     const int y0 = yt * g.ty;
@@ -215,7 +217,7 @@ stencil_cfd_kernel(const V* __restrict__ in, V* __restrict__ out,
             if (!act[k]) continue;
             const int row = yl[k] + R;
             const int z = zz[k];
-            V acc = vscale(q[k][R], cf.c0);
+            V acc = vscale(q[k][R], c0b);
 #pragma unroll
             for (int j = 0; j < R; j++) {
                 acc = fma_rs(cf.cm[0][j], q[k][R - 1 - j], acc);
@@ -351,6 +353,7 @@ stencil_ring_kernel(const V* __restrict__ in, V* __restrict__ out,
     if (rd > 0) ring_barrier();   // last reads of the previous tile done
     int band, xc, yt;
     ring_block_coords(rg, L, band, xc, yt);
+    const S c0b = cf.eps ? cf.c0 - (S)cf.eps[band] : cf.c0;
     const int y0 = yt * g.ty;
     const int tye = min(g.ty, g.n1 - y0);
     const int x0 = xc * g.lx;
@@ -470,7 +473,7 @@ stencil_ring_kernel(const V* __restrict__ in, V* __restrict__ out,
         // is full in this loop and would spill them through v_readlane)
         // (symmetric stencils only: with both signs pinned the register
         // file overflows)
-        S k0 = cf.c0;
+        S k0 = c0b;
         S km[3][R], kp[3][R];
         if (SYM) asm volatile("" : "+v"(k0));
 #pragma unroll
@@ -603,7 +606,7 @@ stencil_ring_kernel(const V* __restrict__ in, V* __restrict__ out,
         }
         // (symmetric stencils only: with both signs pinned the register
         // file overflows)
-        S k0 = cf.c0;
+        S k0 = c0b;
         S km[3][R], kp[3][R];
         if (SYM) asm volatile("" : "+v"(k0));
 #pragma unroll
@@ -775,7 +778,7 @@ stencil_ring_kernel(const V* __restrict__ in, V* __restrict__ out,
             if (!act[k]) continue;
             const int row = yl[k] + R;
             const int z = zz[k];
-            V acc = vscale(q[k][R], cf.c0);
+            V acc = vscale(q[k][R], c0b);
 #pragma unroll
             for (int j = 0; j < R; j++) {
                 acc = fma_rs(cf.cm[0][j], q[k][R - 1 - j], acc);
@@ -965,7 +968,7 @@ template <typename V, typename S, int R, int PPT>
 int launch_cfd_r(const gh_stencil* h, const void* in, void* out,
                  int64_t nbands, const double* phase_cd, const void* vt,
                  const void* src, double beta, double alpha, double gamma,
-                 hipStream_t st) {
+                 const double* eps, hipStream_t st) {
     TileChoice t = choose_tile(h, sizeof(V), PPT);
     GH_REQUIRE(t.threads <= kCfdMaxThreads && t.lds <= 64 * 1024, GH_ERR_UNSUPPORTED,
                "stencil: grid rows of %lld points do not fit one block",
@@ -978,6 +981,7 @@ int launch_cfd_r(const gh_stencil* h, const void* in, void* out,
     g.nxc = (g.n0 + t.lx - 1) / t.lx;
     CfdCoefs<S> cf;
     cf.c0 = (S)(alpha * h->c0 + gamma);
+    cf.eps = eps;
     for (int c = 0; c < 3; c++)
         for (int j = 0; j < kMaxR; j++) {
             cf.cm[c][j] = (S)(alpha * h->cm[c][j]);
@@ -1019,7 +1023,7 @@ template <typename V, typename S, int R, int PPT>
 int launch_ring_r(const gh_stencil* h, const void* in, void* out,
                   int64_t nbands, const double* phase_cd, const void* vt,
                   const void* src, double beta, double alpha, double gamma,
-                  hipStream_t st) {
+                  const double* eps, hipStream_t st) {
     const int n0 = (int)h->n[0], n1 = (int)h->n[1], n2 = (int)h->n[2];
     const int vs = (int)sizeof(V);
     if ((n2 * vs) % kDmaBytes != 0) return GH_ERR_UNSUPPORTED;
@@ -1116,6 +1120,7 @@ int launch_ring_r(const gh_stencil* h, const void* in, void* out,
     }
     CfdCoefs<S> cf;
     cf.c0 = (S)(alpha * h->c0 + gamma);
+    cf.eps = eps;
     for (int c = 0; c < 3; c++)
         for (int j = 0; j < kMaxR; j++) {
             cf.cm[c][j] = (S)(alpha * h->cm[c][j]);
@@ -1174,7 +1179,7 @@ template <> struct RingPPT<cplx<float>> { static constexpr int value = 2; };
 template <typename V, typename S, int PPT>
 int launch_cfd(const gh_stencil* h, const void* in, void* out, int64_t nb,
                const double* ph, const void* vt, const void* src, double beta,
-               double al, double ga, hipStream_t st) {
+               double al, double ga, const double* eps, hipStream_t st) {
     if (!h->reg_only) {
         constexpr int RP = RingPPT<V>::value;
         int rc = GH_ERR_UNSUPPORTED;
@@ -1187,26 +1192,26 @@ int launch_cfd(const gh_stencil* h, const void* in, void* out, int64_t nb,
                     two = two && ph[(c * 2 + d) * 2] == 1.0 && ph[(c * 2 + d) * 2 + 1] == 0.0;
         if (two) {
             switch (h->R) {
-                case 1: rc = launch_ring_r<V, S, 1, RP>(h, in, out, nb, ph, vt, src, beta, al, ga, st); break;
-                case 2: rc = launch_ring_r<V, S, 2, RP>(h, in, out, nb, ph, vt, src, beta, al, ga, st); break;
-                case 3: rc = launch_ring_r<V, S, 3, RP>(h, in, out, nb, ph, vt, src, beta, al, ga, st); break;
+                case 1: rc = launch_ring_r<V, S, 1, RP>(h, in, out, nb, ph, vt, src, beta, al, ga, eps, st); break;
+                case 2: rc = launch_ring_r<V, S, 2, RP>(h, in, out, nb, ph, vt, src, beta, al, ga, eps, st); break;
+                case 3: rc = launch_ring_r<V, S, 3, RP>(h, in, out, nb, ph, vt, src, beta, al, ga, eps, st); break;
             }
         } else {
             switch (h->R) {
-                case 1: rc = launch_ring_r<V, S, 1, 1>(h, in, out, nb, ph, vt, src, beta, al, ga, st); break;
-                case 2: rc = launch_ring_r<V, S, 2, 1>(h, in, out, nb, ph, vt, src, beta, al, ga, st); break;
-                case 3: rc = launch_ring_r<V, S, 3, 1>(h, in, out, nb, ph, vt, src, beta, al, ga, st); break;
+                case 1: rc = launch_ring_r<V, S, 1, 1>(h, in, out, nb, ph, vt, src, beta, al, ga, eps, st); break;
+                case 2: rc = launch_ring_r<V, S, 2, 1>(h, in, out, nb, ph, vt, src, beta, al, ga, eps, st); break;
+                case 3: rc = launch_ring_r<V, S, 3, 1>(h, in, out, nb, ph, vt, src, beta, al, ga, eps, st); break;
             }
         }
         if (rc != GH_ERR_UNSUPPORTED) return rc;
     }
     switch (h->R) {
-        case 1: return launch_cfd_r<V, S, 1, PPT>(h, in, out, nb, ph, vt, src, beta, al, ga, st);
-        case 2: return launch_cfd_r<V, S, 2, PPT>(h, in, out, nb, ph, vt, src, beta, al, ga, st);
-        case 3: return launch_cfd_r<V, S, 3, PPT>(h, in, out, nb, ph, vt, src, beta, al, ga, st);
-        case 4: return launch_cfd_r<V, S, 4, PPT>(h, in, out, nb, ph, vt, src, beta, al, ga, st);
-        case 5: return launch_cfd_r<V, S, 5, PPT>(h, in, out, nb, ph, vt, src, beta, al, ga, st);
-        case 6: return launch_cfd_r<V, S, 6, PPT>(h, in, out, nb, ph, vt, src, beta, al, ga, st);
+        case 1: return launch_cfd_r<V, S, 1, PPT>(h, in, out, nb, ph, vt, src, beta, al, ga, eps, st);
+        case 2: return launch_cfd_r<V, S, 2, PPT>(h, in, out, nb, ph, vt, src, beta, al, ga, eps, st);
+        case 3: return launch_cfd_r<V, S, 3, PPT>(h, in, out, nb, ph, vt, src, beta, al, ga, eps, st);
+        case 4: return launch_cfd_r<V, S, 4, PPT>(h, in, out, nb, ph, vt, src, beta, al, ga, eps, st);
+        case 5: return launch_cfd_r<V, S, 5, PPT>(h, in, out, nb, ph, vt, src, beta, al, ga, eps, st);
+        case 6: return launch_cfd_r<V, S, 6, PPT>(h, in, out, nb, ph, vt, src, beta, al, ga, eps, st);
     }
     set_error("stencil: radius %d unsupported", h->R);
     return GH_ERR_UNSUPPORTED;
@@ -1345,6 +1350,16 @@ int gh_stencil_apply(const gh_stencil* h, const void* in_dev, void* out_dev,
                                     (hipStream_t)stream);
 }
 
+int gh_stencil_apply_shifted(const gh_stencil* h, const void* in_dev,
+                             void* out_dev, int64_t nbands,
+                             const double* phase_cd, const void* vt_dev,
+                             const double* eps_dev, void* stream) {
+    GH_REQUIRE(eps_dev, GH_ERR_ARG, "stencil_apply_shifted: null eps");
+    return gh::stencil_apply_full(h, in_dev, out_dev, nbands, phase_cd, vt_dev,
+                                  nullptr, 0.0, 1.0, 0.0, eps_dev,
+                                  (hipStream_t)stream);
+}
+
 int gh_stencil_relax(const gh_stencil* h, int relax_method, void* f_dev,
                      const void* s_dev, void* work_dev, int nrelax, double w,
                      void* stream) {
@@ -1389,18 +1404,28 @@ int stencil_apply_scaled(const gh_stencil* h, const void* in_dev, void* out_dev,
                          int64_t nbands, const double* phase_cd,
                          const void* vt_dev, const void* src_dev, double beta,
                          double al, double ga, hipStream_t st) {
+    return stencil_apply_full(h, in_dev, out_dev, nbands, phase_cd, vt_dev, src_dev,
+                              beta, al, ga, nullptr, st);
+}
+
+int stencil_apply_full(const gh_stencil* h, const void* in_dev, void* out_dev,
+                       int64_t nbands, const double* phase_cd,
+                       const void* vt_dev, const void* src_dev, double beta,
+                       double al, double ga, const double* eps, hipStream_t st) {
     GH_REQUIRE(h && in_dev && out_dev, GH_ERR_ARG, "stencil_apply: null argument");
     GH_REQUIRE(in_dev != out_dev, GH_ERR_ARG, "stencil_apply: in and out must differ");
     GH_REQUIRE(nbands >= 0, GH_ERR_ARG, "stencil_apply: nbands < 0");
     if (nbands == 0) return GH_OK;
     if (h->cfd) {
         switch (h->dtype) {
-            case GH_F64: return launch_cfd<double, double, 4>(h, in_dev, out_dev, nbands, phase_cd, vt_dev, src_dev, beta, al, ga, st);
-            case GH_C128: return launch_cfd<cplx<double>, double, 2>(h, in_dev, out_dev, nbands, phase_cd, vt_dev, src_dev, beta, al, ga, st);
-            case GH_F32: return launch_cfd<float, float, 8>(h, in_dev, out_dev, nbands, phase_cd, vt_dev, src_dev, beta, al, ga, st);
-            case GH_C64: return launch_cfd<cplx<float>, float, 4>(h, in_dev, out_dev, nbands, phase_cd, vt_dev, src_dev, beta, al, ga, st);
+            case GH_F64: return launch_cfd<double, double, 4>(h, in_dev, out_dev, nbands, phase_cd, vt_dev, src_dev, beta, al, ga, eps, st);
+            case GH_C128: return launch_cfd<cplx<double>, double, 2>(h, in_dev, out_dev, nbands, phase_cd, vt_dev, src_dev, beta, al, ga, eps, st);
+            case GH_F32: return launch_cfd<float, float, 8>(h, in_dev, out_dev, nbands, phase_cd, vt_dev, src_dev, beta, al, ga, eps, st);
+            case GH_C64: return launch_cfd<cplx<float>, float, 4>(h, in_dev, out_dev, nbands, phase_cd, vt_dev, src_dev, beta, al, ga, eps, st);
         }
     } else {
+        GH_REQUIRE(!eps, GH_ERR_UNSUPPORTED,
+                   "stencil_apply: per-band shift needs an orthogonal stencil");
         switch (h->dtype) {
             case GH_F64: return launch_generic<double, double>(h, in_dev, out_dev, nbands, phase_cd, vt_dev, src_dev, beta, al, ga, st);
             case GH_C128: return launch_generic<cplx<double>, double>(h, in_dev, out_dev, nbands, phase_cd, vt_dev, src_dev, beta, al, ga, st);

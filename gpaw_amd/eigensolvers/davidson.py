@@ -135,7 +135,5 @@ class Davidson(Eigensolver):
             wfs.eps = e[:nb].to(torch.float64).contiguous()
             del psit2, new
             if nit < self.niter - 1:
-                self.apply_h_local(ham, wfs, wfs.psit, R)
-                self.calculate_residuals(ham, wfs, wfs.psit, wfs.P,
-                                         wfs.eps, R)
+                self.residual_of(ham, wfs, wfs.psit, wfs.eps, R, P=wfs.P)
         return error

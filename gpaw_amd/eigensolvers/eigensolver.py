@@ -30,8 +30,23 @@ class Eigensolver:
         """R <- R - eps psit + sum_a p_a (P dH_a - eps P dS_a), where R holds
         (T + v) psit on entry (eigensolver.py:153-172)."""
         blas.multi_axpy(-eps, psit, R)
+        self.add_nonlocal_residual(ham, P, eps, R)
+
+    def add_nonlocal_residual(self, ham, P, eps, R):
+        """R += sum_a p_a (P dH_a - eps P dS_a)."""
         if ham.pt is not None:
             ham.pt.add_M(R, ham.dH(P, eps_x=eps))
+
+    def residual_of(self, ham, wfs, x, eps, out, P=None):
+        """out = (H - eps S) x: the local part (T + v - eps) x in one
+        stencil pass, then the projector terms (with P = <p|x> unless
+        given)."""
+        ham.apply_local_shifted(x, out, eps, wfs.phase_cd)
+        if ham.pt is not None:
+            if P is None:
+                P = ham.pt.integrate_M(x)
+            self.add_nonlocal_residual(ham, P, eps, out)
+        return out
 
     def subspace_diagonalize(self, ham, wfs):
         """Rotate psit, P to the eigenvectors of the subspace Hamiltonian;

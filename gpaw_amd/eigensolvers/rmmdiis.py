@@ -109,13 +109,6 @@ class RMMDIIS(Eigensolver):
         self.error = None
 
     # ----------------------------------------------------------- pieces --
-    def residual_of(self, ham, wfs, x, eps, out):
-        """out = (H - eps S) x."""
-        self.apply_h_local(ham, wfs, x, out)
-        P = ham.pt.integrate_M(x) if ham.pt is not None else None
-        self.calculate_residuals(ham, wfs, x, P, eps, out)
-        return out
-
     # ------------------------------------------------------------ driver --
     def iterate(self, ham, wfs, weights=None):
         """One sweep over all bands (Eigensolver.iterate for one k-point).

@@ -99,6 +99,36 @@ class FDOperator:
                 src.data_ptr() if src is not None else None, float(beta),
                 stream_ptr()))
 
+    def apply_shifted(self, in_xg, out_xg, eps_x, phase_cd=None, vt_g=None):
+        """out[x] = (L + vt - eps[x]) in[x] for device arrays: the local part
+        of the residual (H - eps) psi (eigensolver.py:153-172) in one pass;
+        eps_x: per-band FP64 values.  Non-orthogonal stencils run the plain
+        apply followed by a band-wise axpy."""
+        assert tuple(in_xg.shape) == tuple(out_xg.shape)
+        assert tuple(in_xg.shape[-3:]) == self.shape
+        assert in_xg.is_cuda and out_xg.is_cuda
+        assert in_xg.dtype == out_xg.dtype == torch_dtype(self.dtype)
+        assert in_xg.is_contiguous() and out_xg.is_contiguous()
+        nb = int(np.prod(in_xg.shape[:-3], dtype=np.int64))
+        eps = torch.as_tensor(eps_x, dtype=torch.float64,
+                              device='cuda').reshape(-1).contiguous()
+        assert eps.numel() == nb
+        if not self.cfd:
+            from . import blas
+            self.apply(in_xg, out_xg, phase_cd, vt_g=vt_g)
+            blas.multi_axpy(-eps, in_xg.reshape(nb, -1),
+                            out_xg.reshape(nb, -1))
+            return
+        ph, php = _lib.phase_arg(phase_cd)
+        vt = None
+        if vt_g is not None:
+            vt = Staged(vt_g, dtype=real_dtype(self.dtype)).t
+            assert tuple(vt.shape) == self.shape
+        _lib.check(self._lib.gh_stencil_apply_shifted(
+            self.handle, in_xg.data_ptr(), out_xg.data_ptr(), nb, php,
+            vt.data_ptr() if vt is not None else None, eps.data_ptr(),
+            stream_ptr()))
+
     def __call__(self, in_xR, out_xR=None):
         if out_xR is None:
             out_xR = in_xR.new_empty(in_xR.shape) if hasattr(

@@ -74,6 +74,11 @@ class FDHamiltonian:
         """out = T psi + v psi (one fused stencil pass)."""
         self.kin.apply(psit_xG, out_xG, phase_cd, vt_g=self.vt_G)
 
+    def apply_local_shifted(self, psit_xG, out_xG, eps_x, phase_cd=None):
+        """out = (T + v - eps_x) psi per band (one fused stencil pass)."""
+        self.kin.apply_shifted(psit_xG, out_xG, eps_x, phase_cd,
+                               vt_g=self.vt_G)
+
     def apply(self, psit_xG, out_xG, phase_cd=None, P_xM=None):
         """out = H psi; returns the projections P = <p|psi>."""
         self.apply_local(psit_xG, out_xG, phase_cd)

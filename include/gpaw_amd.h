@@ -79,6 +79,18 @@ int gh_stencil_apply(const gh_stencil* h, const void* in_dev, void* out_dev,
                      const void* vt_dev, const void* src_dev, double beta,
                      void* stream);
 
+/* out[x] = (L + vt - eps[x]) in[x] per band: the local part of the
+ * residual R = (H - eps S) psi in one stencil pass -- replaces
+ * apply_pseudo_hamiltonian (wavefunctions/fd.py:82-88) followed by the
+ * -eps psi axpy of Eigensolver.calculate_residuals (eigensolver.py:153-172;
+ * the non-local part is added by gh_lfc_add).  eps_dev: nbands doubles on
+ * the device.  vt_dev may be null.  Orthogonal-cell stencils only
+ * (GH_ERR_UNSUPPORTED otherwise). */
+int gh_stencil_apply_shifted(const gh_stencil* h, const void* in_dev,
+                             void* out_dev, int64_t nbands,
+                             const double* phase_cd, const void* vt_dev,
+                             const double* eps_dev, void* stream);
+
 /* Tile choice of the stencil kernel: rows of the (y, z) tile per block,
  * planes per block along x, and the prefetch depth in planes (2 or 3) of
  * the LDS-ring kernel; 0 = automatic.  Used by the autotuner. */

@@ -129,6 +129,43 @@ def test_stencil_kernel_variants(monkeypatch, kernel, N_c, pbc, dtype, k_c):
         assert relerr(host(out), ref) < tol, (rows, planes)
 
 
+@pytest.mark.parametrize('kernel', ['ring', 'reg'])
+@pytest.mark.parametrize('dtype,k_c,cell', [
+    (np.float64, None, 'ortho'), (np.complex128, (0.25, 0, 0), 'ortho'),
+    (np.complex128, (0.1, -0.3, 0.45), 'ortho'), (np.float32, None, 'ortho'),
+    (np.complex64, (0.25, 0, 0), 'ortho'), (np.complex128, (0.25, 0.1, 0), 'fcc')])
+def test_apply_shifted_residual(monkeypatch, kernel, dtype, k_c, cell):
+    """(T + v - eps_x) psi in one pass against the oracle, every kernel path
+    (two-point real and complex, one-point complex, register-queue; the
+    non-orthogonal stencil takes apply + axpy)."""
+    monkeypatch.delenv('GPAW_AMD_STENCIL', raising=False)
+    if kernel == 'reg':
+        monkeypatch.setenv('GPAW_AMD_STENCIL', 'reg')
+    rng = np.random.default_rng(16)
+    N_c = (16, 20, 24)
+    cell_cv = (4.0 * np.array([(0, 1, 1), (1, 0, 1), (1, 1, 0)]) if cell == 'fcc'
+               else np.diag(np.array(N_c) * 0.3))
+    gd = GridDescriptor(N_c, cell_cv)
+    ogd = og.Grid(N_c, cell_cv)
+    nb = 5
+    a = rand(rng, (nb,) + N_c, dtype)
+    single = dtype in (np.float32, np.complex64)
+    vt = rng.standard_normal(N_c).astype(np.float32 if single else np.float64)
+    eps = rng.uniform(-1, 1, nb)
+    ph = None if k_c is None else ogd.phase_cd(k_c)
+    op = Laplace(gd, -0.5, 2 if cell == 'fcc' else 3, dtype)
+    out = torch.empty_like(dev(a))
+    op.apply_shifted(dev(a), out, torch.from_numpy(eps).cuda(), ph,
+                     vt_g=dev(vt))
+    c, o = ost.laplace_stencil(ogd.h_cv, -0.5, 2 if cell == 'fcc' else 3)
+    wide = np.complex128 if np.dtype(dtype).kind == 'c' else np.float64
+    aw = a.astype(wide)
+    ref = (ost.fd_apply(c, o, aw, ogd.pbc_c, ph) + vt * aw
+           - eps[:, None, None, None] * aw)
+    tol = 2e-5 if single else 1e-12
+    assert relerr(host(out), ref) < tol
+
+
 def test_laplace_fused_potential_and_source():
     rng = np.random.default_rng(2)
     N_c = (24, 20, 28)
