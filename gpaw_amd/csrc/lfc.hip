@@ -34,7 +34,7 @@ namespace gh {
 
 constexpr int kLfcMaxNm = 64;   // functions per atom supported
 constexpr int kIntChunk = 256;  // entries staged per integrate step
-constexpr int kAddBT = 32;      // bands per add wavefront
+constexpr int kAddBT = 32;      // bands per add wavefront (at most)
 constexpr int kAddGrp = 32;     // bands loaded together in add
 constexpr int kAddChunk = 64;   // entries per add wavefront (one per lane)
 
@@ -132,7 +132,7 @@ lfc_integrate_kernel(const V* __restrict__ a, V* __restrict__ c,
 }
 
 // add: one wavefront per (unit of the colour, chunk of 64 entries, band
-// tile of kAddBT bands).  A unit is a contiguous entry range of one atom in
+// tile of bt <= kAddBT bands).  A unit is a contiguous entry range of one atom in
 // which every grid point occurs once (an atom whose sphere overlaps its own
 // periodic image has several units).  lane = entry: the lane keeps its
 // entry's grid index, phase and function values (registers when the count
@@ -152,18 +152,18 @@ lfc_add_kernel(const V* __restrict__ c, V* __restrict__ a,
                const int32_t* __restrict__ G_e,
                const int32_t* __restrict__ w_e, const S* __restrict__ A,
                const S* __restrict__ phase_w, int64_t ngpts, int Mmax,
-               int nbands) {
+               int nbands, int bt) {
     const int unit = units[blockIdx.x / nchunks];
     const int chunk = blockIdx.x % nchunks;
     const int atom = unit_atom[unit];
-    const int x0 = blockIdx.y * kAddBT;
+    const int x0 = blockIdx.y * bt;
     const int64_t e0 = entry_ptr[atom];
     const int64_t eb = unit_beg[unit] + (int64_t)chunk * kAddChunk;
     const int64_t e1 = min(unit_end[unit], eb + kAddChunk);
     if (eb >= e1) return;
     const int M0 = M_ptr[atom];
     const int nm = NM > 0 ? NM : M_ptr[atom + 1] - M0;
-    const int nb = min(kAddBT, nbands - x0);
+    const int nb = min(bt, nbands - x0);
     const int64_t e = eb + threadIdx.x;
     if (e >= e1) return;
     const int64_t G = G_e[e];
@@ -498,18 +498,22 @@ int gh_lfc_add(const gh_lfc* h, const void* c_dev, void* a_dev,
     if (nbands == 0 || h->nentries == 0) return GH_OK;
     hipStream_t st = (hipStream_t)stream;
     const bool ph = cplxd(h->dtype) && h->phase_w;
-    const unsigned ntiles = (unsigned)((nbands + kAddBT - 1) / kAddBT);
     const bool uniform8 = h->uniform_nm == 8;
     for (size_t col = 0; col + 1 < h->color_ptr.size(); col++) {
         const int n = h->color_ptr[col + 1] - h->color_ptr[col];
         if (n == 0) continue;
         const int nchunks = (int)((h->max_unit + kAddChunk - 1) / kAddChunk);
+        // small colours take narrower band tiles, so that every launch
+        // keeps the GPU busy (about 8192 wavefronts: 32 per CU)
+        int bt = kAddBT;
+        while (bt > 4 && (int64_t)n * nchunks * ((nbands + bt - 1) / bt) < 8192) bt /= 2;
+        const unsigned ntiles = (unsigned)((nbands + bt - 1) / bt);
         dim3 grid(n * nchunks, ntiles);
         const int32_t* units = h->color_units + h->color_ptr[col];
 #define ARGS(V, S) (const V*)c_dev, (V*)a_dev, units, nchunks, h->unit_atom, h->unit_beg, \
                    h->unit_end, h->entry_ptr, h->M_ptr, h->A_off, \
                    h->G_e, h->w_e, (const S*)h->A, ph ? (const S*)h->phase_w : nullptr, \
-                   h->ngpts, h->Mmax, (int)nbands
+                   h->ngpts, h->Mmax, (int)nbands, bt
         switch (h->dtype) {
             case GH_F64:
                 if (uniform8) hipLaunchKernelGGL((lfc_add_kernel<double, double, 8>), grid, dim3(64), 0, st, ARGS(double, double));
