@@ -504,9 +504,9 @@ int gh_lfc_add(const gh_lfc* h, const void* c_dev, void* a_dev,
         if (n == 0) continue;
         const int nchunks = (int)((h->max_unit + kAddChunk - 1) / kAddChunk);
         // small colours take narrower band tiles, so that every launch
-        // keeps the GPU busy (about 8192 wavefronts: 32 per CU)
+        // keeps the GPU busy (about 16384 wavefronts)
         int bt = kAddBT;
-        while (bt > 4 && (int64_t)n * nchunks * ((nbands + bt - 1) / bt) < 8192) bt /= 2;
+        while (bt > 4 && (int64_t)n * nchunks * ((nbands + bt - 1) / bt) < 16384) bt /= 2;
         const unsigned ntiles = (unsigned)((nbands + bt - 1) / bt);
         dim3 grid(n * nchunks, ntiles);
         const int32_t* units = h->color_units + h->color_ptr[col];
