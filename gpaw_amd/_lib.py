@@ -40,11 +40,12 @@ _SIGNATURES = {
     'gh_transformer_destroy': (_c.c_int, [P]),
     'gh_transformer_apply': (_c.c_int, [P, P, P, I64, PD, P]),
     'gh_lfc_create': (_c.c_int, [_c.POINTER(P), I64, _c.c_int, PI64, PI32,
-                                 PI32, PI32, PD, _c.c_int, PD, D, _c.c_int]),
+                                 PI32, PI32, PD, _c.c_int, _c.c_int, PD, D,
+                                 _c.c_int]),
     'gh_lfc_destroy': (_c.c_int, [P]),
     'gh_lfc_nfunctions': (_c.c_int, [P]),
-    'gh_lfc_integrate': (_c.c_int, [P, P, P, I64, P]),
-    'gh_lfc_add': (_c.c_int, [P, P, P, I64, P]),
+    'gh_lfc_integrate': (_c.c_int, [P, P, P, I64, _c.c_int, P]),
+    'gh_lfc_add': (_c.c_int, [P, P, P, I64, _c.c_int, P]),
     'gh_lfc_apply_dH': (_c.c_int, [P, P, P, P, P, P, I64, P]),
     'gh_precond_scratch_bytes': (I64, [P, P, P, I64]),
     'gh_precond_apply': (_c.c_int, [P, P, P, P, P, P, P, D, P, P, I64, PD, P,

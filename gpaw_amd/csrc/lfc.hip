@@ -252,7 +252,9 @@ struct gh_lfc {
     int32_t* G_e = nullptr;
     int32_t* w_e = nullptr;
     void* A = nullptr;         // double or float
-    void* phase_w = nullptr;   // (re, im) pairs, double or float
+    void* phase_w = nullptr;   // (re, im) pairs, double or float: nq tables of nw
+    int nq = 0;
+    int nw = 0;
     int max_nm = 0;                     // most functions on one atom
     int uniform_nm = -1;                // functions per atom if all equal
     int64_t max_unit = 0;               // most entries in one add unit
@@ -296,11 +298,13 @@ extern "C" {
 int gh_lfc_create(gh_lfc** out, int64_t ngpts, int natoms,
                   const int64_t* entry_ptr, const int32_t* M_ptr,
                   const int32_t* G_e, const int32_t* w_e, const double* A,
-                  int nw, const double* phase_w, double dv, int dtype) {
+                  int nq, int nw, const double* phase_qw, double dv, int dtype) {
     GH_REQUIRE(out && entry_ptr && M_ptr && natoms >= 0, GH_ERR_ARG,
                "lfc_create: null argument");
     GH_REQUIRE(dtype >= GH_F64 && dtype <= GH_C64, GH_ERR_ARG, "lfc_create: bad dtype");
     GH_REQUIRE(ngpts > 0 && ngpts < (1ll << 31), GH_ERR_SHAPE, "lfc_create: bad ngpts");
+    GH_REQUIRE(nw >= 0 && (!phase_qw || nq >= 1), GH_ERR_ARG,
+               "lfc_create: phase tables need nq >= 1 and nw >= 0");
     const int64_t ne = entry_ptr[natoms];
     GH_REQUIRE(ne == 0 || (G_e && A), GH_ERR_ARG, "lfc_create: missing entries");
     // host-side validation and derived tables
@@ -413,6 +417,8 @@ int gh_lfc_create(gh_lfc** out, int64_t ngpts, int natoms,
         h->max_unit = std::max(h->max_unit, unit_end[u] - unit_beg[u]);
     h->ngpts = ngpts;
     h->nentries = ne;
+    h->nq = phase_qw && nw > 0 ? nq : 0;
+    h->nw = nw;
     h->dv = dv;
     // function values in the permuted entry order
     std::vector<double> A2((size_t)A_off[natoms]);
@@ -441,8 +447,8 @@ int gh_lfc_create(gh_lfc** out, int64_t ngpts, int natoms,
         float* d = nullptr;
         UP(&d, Af.data(), nA);
         h->A = d;
-        if (phase_w && nw > 0) {
-            std::vector<float> pf(phase_w, phase_w + 2 * nw);
+        if (h->nq > 0) {
+            std::vector<float> pf(phase_qw, phase_qw + (size_t)2 * nq * nw);
             float* dp = nullptr;
             UP(&dp, pf.data(), pf.size());
             h->phase_w = dp;
@@ -451,9 +457,9 @@ int gh_lfc_create(gh_lfc** out, int64_t ngpts, int natoms,
         double* d = nullptr;
         UP(&d, A2.data(), nA);
         h->A = d;
-        if (phase_w && nw > 0) {
+        if (h->nq > 0) {
             double* dp = nullptr;
-            UP(&dp, phase_w, (size_t)2 * nw);
+            UP(&dp, phase_qw, (size_t)2 * nq * nw);
             h->phase_w = dp;
         }
     }
@@ -469,18 +475,27 @@ int gh_lfc_destroy(gh_lfc* h) {
 
 int gh_lfc_nfunctions(const gh_lfc* h) { return h ? h->Mmax : 0; }
 
+// Phase table of k-point q (null for real data or no images).
+static const void* phase_table(const gh_lfc* h, int q) {
+    if (!cplxd(h->dtype) || h->nq == 0) return nullptr;
+    const size_t s = single(h->dtype) ? sizeof(float) : sizeof(double);
+    return (const char*)h->phase_w + (size_t)q * h->nw * 2 * s;
+}
+
 int gh_lfc_integrate(const gh_lfc* h, const void* a_dev, void* c_dev,
-                     int64_t nbands, void* stream) {
+                     int64_t nbands, int q, void* stream) {
     GH_REQUIRE(h && a_dev && c_dev, GH_ERR_ARG, "lfc_integrate: null argument");
+    GH_REQUIRE(q >= 0 && (q < h->nq || (h->nq == 0 && q == 0)), GH_ERR_ARG,
+               "lfc_integrate: k-point index %d out of range (%d tables)", q, h->nq);
     if (nbands == 0 || h->natoms == 0) return GH_OK;
     // every (band, M) is written by the block of its atom, also for atoms
     // without grid points, so no clearing pass is needed
     hipStream_t st = (hipStream_t)stream;
     const int bt = h->dtype == GH_C128 ? IntBT<cplx<double>>::value : IntBT<double>::value;
     dim3 grid(h->natoms, (unsigned)((nbands + bt - 1) / bt));
-    const bool ph = cplxd(h->dtype) && h->phase_w;
+    const void* ph = phase_table(h, q);
 #define ARGS(V, S) (const V*)a_dev, (V*)c_dev, h->entry_ptr, h->M_ptr, h->A_off, h->G_e, h->w_e, \
-                   (const S*)h->A, ph ? (const S*)h->phase_w : nullptr, h->ngpts, h->Mmax, (int)nbands, (S)h->dv
+                   (const S*)h->A, (const S*)ph, h->ngpts, h->Mmax, (int)nbands, (S)h->dv
     switch (h->dtype) {
         case GH_F64: hipLaunchKernelGGL((lfc_integrate_kernel<double, double>), grid, dim3(256), 0, st, ARGS(double, double)); break;
         case GH_C128: hipLaunchKernelGGL((lfc_integrate_kernel<cplx<double>, double>), grid, dim3(256), 0, st, ARGS(cplx<double>, double)); break;
@@ -493,11 +508,13 @@ int gh_lfc_integrate(const gh_lfc* h, const void* a_dev, void* c_dev,
 }
 
 int gh_lfc_add(const gh_lfc* h, const void* c_dev, void* a_dev,
-               int64_t nbands, void* stream) {
+               int64_t nbands, int q, void* stream) {
     GH_REQUIRE(h && a_dev && c_dev, GH_ERR_ARG, "lfc_add: null argument");
+    GH_REQUIRE(q >= 0 && (q < h->nq || (h->nq == 0 && q == 0)), GH_ERR_ARG,
+               "lfc_add: k-point index %d out of range (%d tables)", q, h->nq);
     if (nbands == 0 || h->nentries == 0) return GH_OK;
     hipStream_t st = (hipStream_t)stream;
-    const bool ph = cplxd(h->dtype) && h->phase_w;
+    const void* ph = phase_table(h, q);
     const bool uniform8 = h->uniform_nm == 8;
     for (size_t col = 0; col + 1 < h->color_ptr.size(); col++) {
         const int n = h->color_ptr[col + 1] - h->color_ptr[col];
@@ -512,7 +529,7 @@ int gh_lfc_add(const gh_lfc* h, const void* c_dev, void* a_dev,
         const int32_t* units = h->color_units + h->color_ptr[col];
 #define ARGS(V, S) (const V*)c_dev, (V*)a_dev, units, nchunks, h->unit_atom, h->unit_beg, \
                    h->unit_end, h->entry_ptr, h->M_ptr, h->A_off, \
-                   h->G_e, h->w_e, (const S*)h->A, ph ? (const S*)h->phase_w : nullptr, \
+                   h->G_e, h->w_e, (const S*)h->A, (const S*)ph, \
                    h->ngpts, h->Mmax, (int)nbands, bt
         switch (h->dtype) {
             case GH_F64:

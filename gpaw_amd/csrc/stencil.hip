@@ -1027,6 +1027,13 @@ int launch_ring_r(const gh_stencil* h, const void* in, void* out,
     const int n0 = (int)h->n[0], n1 = (int)h->n[1], n2 = (int)h->n[2];
     const int vs = (int)sizeof(V);
     if ((n2 * vs) % kDmaBytes != 0) return GH_ERR_UNSUPPORTED;
+    // the plane DMA and the paired loads / stores move 16-byte vectors: a
+    // pointer off that alignment (a view at an odd element offset) takes the
+    // register kernel instead
+    auto aligned16 = [](const void* p) { return ((uintptr_t)p & 15u) == 0; };
+    if (!aligned16(in) || !aligned16(out) || (vt && !aligned16(vt)) ||
+        (src && !aligned16(src)))
+        return GH_ERR_UNSUPPORTED;
     auto lds_of = [&](int c, int d) {
         return (size_t)(R + d + 1) * (c + 2 * R) * n2 * vs;
     };

@@ -2,11 +2,11 @@
 
 FDOperator / Laplace / GUCLaplace keep the reference's constructor and
 method signatures (gpaw/fd_operators.py:36-141,171-232).  The stencil runs
-in libgpawhip (gh_stencil_*); coefficients are derived on the host exactly as
-GUCLaplace does (least squares over the 13 shortest lattice directions,
-Chelikowsky expansion coefficients).
+in libgpawhip (gh_stencil_*); the Laplacian's points and coefficients are
+derived on the host (guc_laplace_stencil) and agree with the reference's.
 """
 import ctypes
+import itertools
 
 import numpy as np
 import torch
@@ -172,34 +172,72 @@ def Laplace(gd, scale=1.0, n=1, dtype=float, xp=None):
     return GUCLaplace(gd, scale, n, dtype, xp=xp)
 
 
+def _neighbour_directions():
+    """The 13 directions d of the 3 x 3 x 3 grid-point cube with d > 0
+    lexicographically (one of each +-d pair), in lexicographic order."""
+    return np.array([d for d in itertools.product((-1, 0, 1), repeat=3)
+                     if d > (0, 0, 0)])
+
+
+def _second_moments(t_dv):
+    """Row d: the six independent entries (xx, yy, zz, yz, xz, xy) of the
+    outer product t_d t_d^T of the Cartesian step vector t_d."""
+    x, y, z = t_dv.T
+    return np.stack([x * x, y * y, z * z, y * z, x * z, x * y], axis=1)
+
+
 def guc_laplace_stencil(h_cv, scale=1.0, n=1):
-    """Coefficients and offsets of GUCLaplace (fd_operators.py:178-232)."""
-    M_ic = np.indices((3, 3, 3)).reshape((3, -1)).T[-13:] - 1
-    u_cv = h_cv / (h_cv ** 2).sum(1)[:, None] ** 0.5
-    u2_i = (np.dot(M_ic, u_cv) ** 2).sum(1)
-    i_d = u2_i.argsort()
-    m_mv = np.array([(2, 0, 0), (0, 2, 0), (0, 0, 2),
-                     (0, 1, 1), (1, 0, 1), (1, 1, 0)])
-    for D in range(3, 7):
-        h_dv = np.dot(M_ic[i_d[:D]], h_cv)
-        A_md = (h_dv ** m_mv[:, None, :]).prod(2)
-        a_d, residual, rank, s = np.linalg.lstsq(A_md, [1, 1, 1, 0, 0, 0],
-                                                 rcond=-1)
-        if residual.sum() < 1e-14:
-            if rank != D:
-                raise ValueError('You have a weird unit cell!  Try to use the '
-                                 'maximally reduced Niggli cell.')
-            break
+    """Coefficients and offsets of the order-2n Laplacian on a grid with
+    step vectors h_cv (the stencil of fd_operators.py:171-232).
+
+    Along a grid direction d with Cartesian step t_d, the 1-D expansion
+    w_0 f(r) + sum_j w_j (f(r + j t_d) + f(r - j t_d)) (Chelikowsky weights)
+    approximates (t_d . grad)^2 f.  Summing directions with weights a_d gives
+    the Laplacian when sum_d a_d t_d t_d^T = 1 (3 x 3).  Directions are taken
+    shortest first, the length measured with every grid axis scaled to unit
+    length and ranked by NumPy's default argsort, as the reference ranks them
+    (so equal lengths come in that sort's order, which is not always the
+    lexicographic one: the body-centred cubic cell); the first D = 3..6
+    of them for which the six second-moment equations have an exact
+    solution are used.  An orthorhombic cell needs only the three axes, with
+    a_d = 1 / h_d^2.
+
+    Returns (coefs, offsets, D): the centre first, then per direction the
+    points +1..+n and -1..-n."""
+    h_cv = np.asarray(h_cv, float)
+    dirs = _neighbour_directions()
+    unit_cv = h_cv / np.linalg.norm(h_cv, axis=1)[:, None]
+    # equal lengths must compare equal, so the key is formed as a plain sum
+    # of squared components (x^2 + y^2) + z^2
+    rank_key = np.sum((dirs @ unit_cv) ** 2, axis=1)
+    order = np.argsort(rank_key)
+    if np.count_nonzero(h_cv - np.diag(np.diag(h_cv))) == 0:
+        ndir = 3
+        used = dirs[order[:3]]
+        a_d = 1.0 / np.einsum('dv,dv->d', used @ h_cv, used @ h_cv)
+    else:
+        target = np.array([1.0, 1.0, 1.0, 0.0, 0.0, 0.0])
+        for ndir in range(3, 7):
+            used = dirs[order[:ndir]]
+            B = _second_moments(used @ h_cv).T          # (6, ndir)
+            a_d = np.linalg.lstsq(B, target, rcond=None)[0]
+            if np.sum((B @ a_d - target) ** 2) < 1e-14:
+                if np.linalg.matrix_rank(B) < ndir:
+                    raise ValueError(
+                        'grid directions are degenerate for this cell; '
+                        'build the Laplacian on a reduced (Niggli) cell')
+                break
+        else:
+            raise ValueError('no finite-difference Laplacian for this cell')
+    w = np.asarray(laplace[n], float)
     a_d = a_d * scale
-    offsets = [(0, 0, 0)]
-    coefs = [laplace[n][0] * a_d.sum()]
-    for d in range(D):
-        M_c = M_ic[i_d[d]]
-        offsets.extend(np.arange(1, n + 1)[:, None] * M_c)
-        coefs.extend(a_d[d] * np.array(laplace[n][1:]))
-        offsets.extend(np.arange(-1, -n - 1, -1)[:, None] * M_c)
-        coefs.extend(a_d[d] * np.array(laplace[n][1:]))
-    return np.array(coefs), np.array(offsets), D
+    steps = np.arange(1, n + 1)[:, None]
+    coefs = [w[0] * a_d.sum()]
+    offsets = [np.zeros((1, 3), int)]
+    for a, d in zip(a_d, used):
+        coefs.extend(np.concatenate([a * w[1:], a * w[1:]]))
+        offsets.extend([steps * d, -steps * d])
+    return np.array(coefs), np.concatenate(offsets), ndir
 
 
 class GUCLaplace(FDOperator):
@@ -208,6 +246,7 @@ class GUCLaplace(FDOperator):
         kinetic energy operator, n the range (error O(h^2n))."""
         coefs, offsets, D = guc_laplace_stencil(gd.h_cv, scale, n)
         FDOperator.__init__(self, coefs, offsets, gd, dtype, xp=xp)
+        self.ndirections = D
         self.description = (
-            '%d*%d+1=%d point O(h^%d) finite-difference Laplacian' %
-            ((self.npoints - 1) // n, n, self.npoints, 2 * n))
+            'finite-difference Laplacian: %d points (%d directions x 2 x %d '
+            '+ centre), error O(h^%d)' % (self.npoints, D, n, 2 * n))

@@ -7,6 +7,8 @@ refine (:328-341), get_boxes (:344-411) and array constructors that return
 device tensors.  Lengths are in Bohr.  On a non-periodic axis the first grid
 point is excluded (n_c = N_c - 1, beg_c = 1) as in the reference.
 """
+import itertools
+
 import numpy as np
 import torch
 
@@ -87,48 +89,45 @@ class GridDescriptor:
                       self.h_cv).T.copy()
 
     def get_boxes(self, spos_c, rcut, cut=True):
-        """Boxes of grid points enclosing a sphere, one per periodic image
-        (grid_descriptor.py:344-411)."""
-        N_c = self.N_c
-        ncut = rcut * (self.icell_cv ** 2).sum(axis=1) ** 0.5 * N_c
-        npos_c = np.asarray(spos_c) * N_c
-        beg_c = np.ceil(npos_c - ncut).astype(int)
-        end_c = np.ceil(npos_c + ncut).astype(int)
-        for c in range(3):
-            if not self.pbc_c[c]:
-                if cut:
-                    beg_c[c] = max(beg_c[c], 0)
-                    end_c[c] = min(end_c[c], N_c[c])
-                elif beg_c[c] < 0 or end_c[c] > N_c[c]:
-                    raise GridBoundsError(
-                        'Box at %s crosses boundary' % (tuple(spos_c),))
-        range_c = ([], [], [])
-        for c in range(3):
-            b = beg_c[c]
-            e = b
-            while e < end_c[c]:
-                b0 = b % N_c[c]
-                e = min(end_c[c], b + N_c[c] - b0)
-                b1 = b + self.beg_c[c] - b0 if b0 < self.beg_c[c] else b
-                e0 = b0 - b + e
-                e1 = e - (e0 - self.end_c[c]) if e0 > self.end_c[c] else e
-                if e1 > b1:
-                    range_c[c].append((b1, e1))
-                b = e
-        boxes = []
-        for b0, e0 in range_c[0]:
-            for b1, e1 in range_c[1]:
-                for b2, e2 in range_c[2]:
-                    b = np.array((b0, b1, b2))
-                    e = np.array((e0, e1, e2))
-                    bb = b % N_c
-                    ee = bb + e - b
-                    disp = (b - bb) // N_c
-                    bb = np.maximum(bb, self.beg_c)
-                    ee = np.minimum(ee, self.end_c)
-                    if (bb < ee).all():
-                        boxes.append((bb, ee, disp))
-        return boxes
+        """Boxes of grid points that may lie within rcut of the scaled
+        position spos_c, one box per periodic image the sphere reaches
+        (the contract of grid_descriptor.py:344-411).
+
+        Each box is (beg_c, end_c, disp_c): a half-open index range of this
+        domain and the image (in cells, along each axis) that the sphere
+        centre must be shifted by to cover it.  Along axis c the sphere spans
+        the unwrapped index range [ceil(x - r), ceil(x + r)) with x = spos_c
+        N_c and r = rcut / (distance between lattice planes) grid steps; the
+        range is cut at multiples of N_c into one piece per image, and each
+        piece is clipped to the domain.  Boxes come in the order of the
+        images, axis 0 slowest."""
+        spos_c = np.asarray(spos_c, float)
+        steps_c = rcut * np.sqrt((self.icell_cv ** 2).sum(axis=1)) * self.N_c
+        lo_c = np.ceil(spos_c * self.N_c - steps_c).astype(int)
+        hi_c = np.ceil(spos_c * self.N_c + steps_c).astype(int)
+        for c in np.flatnonzero(~self.pbc_c):
+            if not cut and (lo_c[c] < 0 or hi_c[c] > self.N_c[c]):
+                raise GridBoundsError(
+                    'sphere at scaled position %s reaches beyond the '
+                    'non-periodic boundary of axis %d' % (tuple(spos_c), c))
+            lo_c[c] = max(lo_c[c], 0)
+            hi_c[c] = min(hi_c[c], self.N_c[c])
+        pieces_c = [self._image_pieces(lo_c[c], hi_c[c], c) for c in range(3)]
+        return [(np.array(beg), np.array(end), np.array(disp))
+                for beg, end, disp in
+                (zip(*combo) for combo in itertools.product(*pieces_c))]
+
+    def _image_pieces(self, lo, hi, c):
+        """[(beg, end, image)] of the unwrapped range [lo, hi) along axis c,
+        one piece per periodic image, clipped to [beg_c, end_c)."""
+        n = int(self.N_c[c])
+        out = []
+        for image in range(lo // n, -(-hi // n)):
+            beg = max(lo - image * n, int(self.beg_c[c]))
+            end = min(hi - image * n, int(self.end_c[c]))
+            if beg < end:
+                out.append((beg, end, image))
+        return out
 
 
 class GridBoundsError(ValueError):

@@ -149,21 +149,22 @@ class LocalizedFunctionsCollection:
         self.npoints_union = len(np.unique(G_e)) if len(G_e) else 0
         complex_ = is_complex(self.dtype)
         if complex_:
-            k_c = self.ibzk_qc[0]
-            phase_w = np.exp(2j * pi * sdisp_Wc.dot(k_c))
+            # one table per k-point (phase_qW, lfc.py:339-340)
+            phase_qw = np.exp(2j * pi * self.ibzk_qc.dot(sdisp_Wc.T))
         else:
             if not self.gamma and self.ibzk_qc.any():
                 raise ValueError('real LFC at a non-Gamma k-point')
-            phase_w = None
+            phase_qw = None
         self._free()
         h = ctypes.c_void_p()
         keep = [_lib.i64(entry_ptr), _lib.i32(M_ptr), _lib.i32(G_e),
                 _lib.i32(w_e), _lib.f64(A)]
-        ph = (np.ascontiguousarray(phase_w, complex)
-              if phase_w is not None else None)
+        ph = (np.ascontiguousarray(phase_qw, complex)
+              if phase_qw is not None else None)
         _lib.check(self._lib.gh_lfc_create(
             ctypes.byref(h), ngpts, natoms, keep[0][1], keep[1][1],
-            keep[2][1], keep[3][1], keep[4][1], len(sdisp_list),
+            keep[2][1], keep[3][1], keep[4][1], len(self.ibzk_qc),
+            len(sdisp_list),
             ph.ctypes.data_as(_lib.PD) if ph is not None else None,
             float(self.gd.dv), dtype_code(self.dtype)))
         self.handle = h
@@ -194,34 +195,63 @@ class LocalizedFunctionsCollection:
             dst_xM[..., M1:M1 + self.Mmax_a[a]] = src_axi[a]
 
     # ------------------------------------------------------ fast paths --
-    def integrate_M(self, a_xG, c_xM=None):
+    def _kpoint(self, q):
+        """Phase-table index of k-point q (q = -1: the only k-point)."""
+        nq = len(self.ibzk_qc)
+        if q == -1:
+            if nq > 1 and is_complex(self.dtype):
+                raise ValueError('q must be given: the LFC has %d k-points'
+                                 % nq)
+            return 0
+        if not 0 <= q < nq:
+            raise ValueError('k-point index %d out of range (%d k-points)'
+                             % (q, nq))
+        return 0 if not is_complex(self.dtype) else int(q)
+
+    def _check_grid(self, a_xG):
+        if tuple(a_xG.shape[-3:]) != tuple(int(n) for n in self.gd.n_c):
+            raise ValueError('array of shape %s does not live on the grid %s'
+                             % (tuple(a_xG.shape), tuple(self.gd.n_c)))
+
+    def integrate_M(self, a_xG, c_xM=None, q=-1):
         """c_xM[x, M] = dv * sum_G a(G) A_M(G) on the GPU."""
+        self._check_grid(a_xG)
         xshape = tuple(a_xG.shape[:-3])
         nb = int(np.prod(xshape, dtype=np.int64))
         if c_xM is None:
             c_xM = torch.empty(xshape + (self.Mmax,),
                                dtype=torch_dtype(self.dtype), device='cuda')
+        elif tuple(c_xM.shape) != xshape + (self.Mmax,):
+            raise ValueError('coefficients of shape %s, expected %s'
+                             % (tuple(c_xM.shape), xshape + (self.Mmax,)))
+        kq = self._kpoint(q)
         with Staged(a_xG, dtype=self.dtype) as a, \
                 Staged(c_xM, writeback=True, dtype=self.dtype) as c:
             _lib.check(self._lib.gh_lfc_integrate(
-                self.handle, a.data_ptr(), c.data_ptr(), nb, stream_ptr()))
+                self.handle, a.data_ptr(), c.data_ptr(), nb, kq,
+                stream_ptr()))
         return c_xM
 
-    def add_M(self, a_xG, c_xM):
+    def add_M(self, a_xG, c_xM, q=-1):
         """a(G) += sum_M c_xM[x, M] A_M(G) on the GPU."""
-        nb = int(np.prod(a_xG.shape[:-3], dtype=np.int64))
+        self._check_grid(a_xG)
+        xshape = tuple(a_xG.shape[:-3])
+        if tuple(c_xM.shape) != xshape + (self.Mmax,):
+            raise ValueError('coefficients of shape %s, expected %s'
+                             % (tuple(c_xM.shape), xshape + (self.Mmax,)))
+        nb = int(np.prod(xshape, dtype=np.int64))
+        kq = self._kpoint(q)
         with Staged(c_xM, dtype=self.dtype) as c, \
                 Staged(a_xG, writeback=True, dtype=self.dtype) as a:
             _lib.check(self._lib.gh_lfc_add(
-                self.handle, c.data_ptr(), a.data_ptr(), nb, stream_ptr()))
+                self.handle, c.data_ptr(), a.data_ptr(), nb, kq,
+                stream_ptr()))
 
     # ---------------------------------------------- reference interface --
     def integrate(self, a_xG, c_axi, q=-1):
         """c_axi[a] = integral of a_xG times the functions of atom a
         (lfc.py:552-631)."""
-        if q == -1:
-            assert not is_complex(self.dtype)
-        c_xM = self.integrate_M(a_xG)
+        c_xM = self.integrate_M(a_xG, q=q)
         for a in self.my_atom_indices:
             if a in c_axi:
                 M1 = self.M_a[a]
@@ -234,8 +264,6 @@ class LocalizedFunctionsCollection:
 
     def add(self, a_xG, c_axi=1.0, q=-1):
         """a_xG += sum_a,i c_axi[a][..., i] Phi_ai (lfc.py:392-472)."""
-        if q == -1:
-            assert not is_complex(self.dtype)
         xshape = tuple(a_xG.shape[:-3])
         td = torch_dtype(self.dtype)
         if isinstance(c_axi, float):
@@ -247,7 +275,7 @@ class LocalizedFunctionsCollection:
                 M1 = self.M_a[a]
                 c_xM[..., M1:M1 + self.Mmax_a[a]] = torch.as_tensor(
                     c_xi, device='cuda')
-        self.add_M(a_xG, c_xM)
+        self.add_M(a_xG, c_xM, q)
 
 
 def LFC(gd, spline_aj, kd=None, cut=False, dtype=float, integral=None,

@@ -138,3 +138,60 @@ def initial_guess(gd, pos_av, nbands, sigma=1.2, seed=1):
             psi[b] = torch.cos(phase + shift)
             b += 1
     return psi
+
+
+def basis_splines(w, sigma=1.2, rcut_gauss=5.0, npts=201):
+    """Atom-centred basis radial functions per atom (an LCAO-like basis):
+    the atom's four projector shapes (l = 0, 0, 1, 1) and an s and a p
+    Gaussian exp(-r^2 / sigma^2) cut at rcut_gauss Bohr -- 12 functions per
+    atom.  Returns [[(l, rcut, f_g)]] in the format of Workload.proj."""
+    r = np.linspace(0, rcut_gauss, npts)
+    g = np.exp(-r**2 / sigma**2)
+    g[-1] = 0.0
+    return [list(p) + [(0, rcut_gauss, g), (1, rcut_gauss, g)]
+            for p in w.proj]
+
+
+def lcao_start(gd, w, H, nbands, dtype=float, k_c=None, basis=None):
+    """LCAO-like start for the iterative eigensolvers, as an SCF starts from
+    the LCAO eigenvectors: the lowest `nbands` Rayleigh-Ritz vectors of the
+    Hamiltonian H (an FDHamiltonian) in the span of atom-centred basis
+    functions (basis_splines by default), evaluated on the grid with
+    periodic images and Bloch phases through an LFC.  The overlap is
+    orthogonalised canonically (eigenvectors with eigenvalues below 1e-9 of
+    the largest are dropped), so near-dependent basis functions are
+    harmless.  Returns device wave functions (nbands, n0, n1, n2),
+    orthonormal (dv sum psi_n^* psi_m = delta_nm)."""
+    import torch
+    from .lfc import LFC
+    from .spline import Spline
+    from ._arrays import torch_dtype
+    basis = basis_splines(w) if basis is None else basis
+    kd = None if k_c is None else np.array([k_c], float)
+    lfc = LFC(gd, [[Spline.from_data(l, rc, f) for l, rc, f in p]
+                   for p in basis], kd, dtype=dtype)
+    lfc.set_positions(w.spos_ac)
+    nbasis = lfc.Mmax
+    if nbasis < nbands:
+        raise ValueError('%d basis functions for %d bands' % (nbasis, nbands))
+    td = torch_dtype(dtype)
+    phi = gd.zeros(nbasis, dtype)
+    lfc.add_M(phi, torch.eye(nbasis, dtype=td, device='cuda'),
+              q=-1 if k_c is None else 0)
+    hphi = torch.empty_like(phi)
+    ph = None if k_c is None else gd.phase_cd(k_c)
+    H.apply(phi, hphi, ph)
+    f = phi.reshape(nbasis, -1)
+    S = gd.dv * (f.conj() @ f.T)
+    if H.pt is not None and H.dS_flat is not None:
+        P = H.pt.integrate_M(phi)
+        S = S + P.conj() @ H.dS(P).T
+    Hm = gd.dv * (f.conj() @ hphi.reshape(nbasis, -1).T)
+    del hphi
+    Hm = 0.5 * (Hm + Hm.conj().T)
+    s, U = torch.linalg.eigh(0.5 * (S + S.conj().T))
+    keep = s > 1e-9 * s.max()
+    X = U[:, keep] / torch.sqrt(s[keep])          # X^H S X = 1
+    e, V = torch.linalg.eigh(X.conj().T @ Hm @ X)
+    C = X @ V[:, :nbands]                          # basis -> band
+    return (C.T @ f).reshape((nbands,) + tuple(gd.n_c)).contiguous()

@@ -73,7 +73,11 @@ class _Transformer:
             if not hasattr(input, 'data_ptr'):
                 output = output.cpu().numpy()
         if tuple(output.shape[-3:]) != self.ngpout:
-            raise ValueError('output grid mismatch')
+            raise ValueError('output grid %s != %s' % (
+                tuple(output.shape[-3:]), self.ngpout))
+        if tuple(output.shape[:-3]) != tuple(input.shape[:-3]):
+            raise ValueError('input holds %s arrays, output %s' % (
+                tuple(input.shape[:-3]), tuple(output.shape[:-3])))
         nb = int(np.prod(input.shape[:-3], dtype=np.int64))
         ph, php = _lib.phase_arg(phases)
         with Staged(input, dtype=self.dtype) as a, \

@@ -70,6 +70,8 @@ double gh_stencil_diagonal(const gh_stencil* h);
 
 /* out = L in  (+ vt .* in if vt_dev) (+ beta * src if src_dev).
  * in_dev/out_dev: (nbands, n0, n1, n2) of the stencil's dtype; in != out.
+ * The fastest kernel needs in, out, vt and src 16-byte aligned and rows of
+ * a multiple of 16 bytes; other pointers and shapes take a slower kernel.
  * vt_dev: local potential (n0, n1, n2), real, same precision as the data
  * (double for F64/C128, float for F32/C64) -- fused
  * Hamiltonian.apply_local_potential (hamiltonian.py:437-443).
@@ -132,18 +134,20 @@ int gh_transformer_apply(const gh_transformer* h, const void* in_dev,
  * The caller describes each atom's functions on its grid points as
  * "entries": for atom a, entries [entry_ptr[a], entry_ptr[a+1]) give the
  * flattened grid index G_e (into n0*n1*n2), the periodic-image index w_e
- * (into phase_w, -1 = no phase) and nm_a = M_ptr[a+1]-M_ptr[a] function
- * values A[e][m] (A_e stored row-major with stride nm_a, atoms concatenated
- * in order).  M_ptr[natoms+1] is the global function numbering
- * (lfc.py:277-331: atom-major, spline-major, m-minor).  phase_w[nw] complex
- * (2 doubles each) is exp(2 pi i k . sdisp_w) (lfc.py:339-340), or NULL for
- * real data.  dv is the grid volume element.  All inputs are host arrays;
- * the handle keeps device copies.
+ * (into the phase tables, -1 = no phase) and nm_a = M_ptr[a+1]-M_ptr[a]
+ * function values A[e][m] (A_e stored row-major with stride nm_a, atoms
+ * concatenated in order).  M_ptr[natoms+1] is the global function numbering
+ * (lfc.py:277-331: atom-major, spline-major, m-minor).  phase_qw[nq][nw]
+ * complex (2 doubles each) is exp(2 pi i k_q . sdisp_w) for each of the nq
+ * k-points (phase_qW, lfc.py:339-340), or NULL for real data.  dv is the
+ * grid volume element.  All inputs are host arrays; the handle keeps device
+ * copies.
  */
 int gh_lfc_create(gh_lfc** out, int64_t ngpts, int natoms,
                   const int64_t* entry_ptr, const int32_t* M_ptr,
                   const int32_t* G_e, const int32_t* w_e, const double* A,
-                  int nw, const double* phase_w, double dv, int dtype);
+                  int nq, int nw, const double* phase_qw, double dv,
+                  int dtype);
 int gh_lfc_destroy(gh_lfc* h);
 int gh_lfc_nfunctions(const gh_lfc* h);
 
@@ -164,14 +168,17 @@ int64_t gh_spline_to_grid(int l, int nbins, double dr, const double* coefs,
                           const int64_t* n_c, const int64_t* gdcorner_c,
                           double* A_out, int32_t* G_out);
 
-/* c[x, M] = dv * sum_G a[x, G] A_M(G) [* phase]   (written, not added)
- * a_dev: (nbands, ngpts); c_dev: (nbands, Mmax), same dtype. */
+/* c[x, M] = dv * sum_G a[x, G] A_M(G) [* phase_q]   (written, not added)
+ * a_dev: (nbands, ngpts); c_dev: (nbands, Mmax), same dtype.  q: k-point
+ * index into the phase tables (0 when there are none), as the q argument of
+ * the reference's LFC.integrate_gpu (c/gpu/lfc.cpp:465-503). */
 int gh_lfc_integrate(const gh_lfc* h, const void* a_dev, void* c_dev,
-                     int64_t nbands, void* stream);
+                     int64_t nbands, int q, void* stream);
 
-/* a[x, G] += sum_M c[x, M] A_M(G) [* conj(phase)]. */
+/* a[x, G] += sum_M c[x, M] A_M(G) [* conj(phase_q)]  (LFC.add_gpu,
+ * lfc.py:416-419). */
 int gh_lfc_add(const gh_lfc* h, const void* c_dev, void* a_dev,
-               int64_t nbands, void* stream);
+               int64_t nbands, int q, void* stream);
 
 /* --------------------------------------------- non-local Hamiltonian --
  * out[x, M_a + i] = sum_j P[x, M_a + j] dH_a[j][i] for every atom

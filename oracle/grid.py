@@ -4,7 +4,7 @@ Follows gpaw/grid_descriptor.py:77-160 (N_c, n_c, beg_c, end_c, h_cv, dv),
 :328-341 (coarsen/refine), :344-411 (get_boxes) and gpaw/domain.py:110-139
 (neighbor_cd, sdisp_cd) for a single domain.
 
-Pinned by: tests/test_oracle_grid.py (dv*N = volume, boxes cover exactly the
+Pinned by: tests/test_grid_boxes.py (boxes cover exactly the
 grid points inside a sphere for brute-force enumeration over periodic images).
 """
 import numpy as np

@@ -1,8 +1,9 @@
 """Finite-difference stencils -- oracle restatement (test infrastructure only).
 
-laplace_stencil  follows fd_operators.py:178-232 (GUCLaplace: lstsq over the 13
-                 shortest lattice directions, Chelikowsky table :21-28, offset
-                 order centre, +1..+n, -1..-n per direction).
+laplace_stencil  follows fd_operators.py:178-232 (GUCLaplace: the 13 shortest
+                 lattice directions, exact second-moment condition, Chelikowsky
+                 table :21-28, offset order centre, +1..+n, -1..-n per
+                 direction).
 pad              follows bc.c:146-258 with one domain: zero everything, paste
                  the interior, then per axis i=0,1,2 copy the low interior slab
                  into the high ghost times phase_cd[i,0] and the high interior
@@ -30,31 +31,44 @@ CHELIKOWSKY = [[0],
 
 
 def laplace_stencil(h_cv, scale=1.0, n=1):
-    """Return (coef_p, offset_pc) of GUCLaplace (fd_operators.py:178-232)."""
+    """Return (coef_p, offset_pc) of GUCLaplace (fd_operators.py:178-232).
+
+    Candidate directions: the 13 points of the 3x3x3 cube after the centre in
+    C order.  They are ranked by |sum_c m_c u_c|^2 with u_c the grid axes
+    scaled to unit length with numpy's default argsort (the reference's
+    ranking, including its order among ties); the first D = 3, 4, 5, 6 are
+    tried until sum_d a_d t_d t_d^T = 1 (t_d = m_d . h_cv) has an exact
+    solution, found here from the normal equations.  Points: the centre,
+    then per direction +1..+n and -1..-n, weights from the Chelikowsky
+    table."""
     h_cv = np.asarray(h_cv, float)
-    M_ic = np.indices((3, 3, 3)).reshape((3, -1)).T[-13:] - 1
-    u_cv = h_cv / (h_cv ** 2).sum(1)[:, None] ** 0.5
-    u2_i = (np.dot(M_ic, u_cv) ** 2).sum(1)
-    i_d = u2_i.argsort()
-    m_mv = np.array([(2, 0, 0), (0, 2, 0), (0, 0, 2),
-                     (0, 1, 1), (1, 0, 1), (1, 1, 0)])
+    cands = [(i // 9 - 1, (i // 3) % 3 - 1, i % 3 - 1) for i in range(14, 27)]
+    u_cv = h_cv / np.sqrt((h_cv ** 2).sum(1))[:, None]
+    key = np.sum((np.array(cands, float) @ u_cv) ** 2, axis=1)
+    ranked = list(np.argsort(key))
+    pairs = [(0, 0), (1, 1), (2, 2), (1, 2), (0, 2), (0, 1)]
+    rhs = np.array([1.0, 1.0, 1.0, 0.0, 0.0, 0.0])
     for D in range(3, 7):
-        h_dv = np.dot(M_ic[i_d[:D]], h_cv)
-        A_md = (h_dv ** m_mv[:, None, :]).prod(2)
-        a_d, residual, rank, s = np.linalg.lstsq(A_md, [1, 1, 1, 0, 0, 0],
-                                                 rcond=-1)
-        if residual.sum() < 1e-14:
-            assert rank == D, 'weird unit cell'
+        t = [np.dot(cands[i], h_cv) for i in ranked[:D]]
+        B = np.array([[tv[p] * tv[q] for tv in t] for p, q in pairs])
+        BtB = B.T @ B
+        if np.linalg.matrix_rank(BtB) < D:
+            if D == 6:
+                raise AssertionError('weird unit cell')
+            continue
+        a_d = np.linalg.solve(BtB, B.T @ rhs)
+        if np.sum((B @ a_d - rhs) ** 2) < 1e-14:
             break
     a_d = a_d * scale
+    w = CHELIKOWSKY[n]
     offsets = [(0, 0, 0)]
-    coefs = [CHELIKOWSKY[n][0] * a_d.sum()]
+    coefs = [w[0] * a_d.sum()]
     for d in range(D):
-        M_c = M_ic[i_d[d]]
-        offsets.extend(np.arange(1, n + 1)[:, None] * M_c)
-        coefs.extend(a_d[d] * np.array(CHELIKOWSKY[n][1:]))
-        offsets.extend(np.arange(-1, -n - 1, -1)[:, None] * M_c)
-        coefs.extend(a_d[d] * np.array(CHELIKOWSKY[n][1:]))
+        m = np.array(cands[ranked[d]])
+        for sign in (1, -1):
+            for j in range(1, n + 1):
+                offsets.append(tuple(sign * j * m))
+                coefs.append(a_d[d] * w[j])
     return np.array(coefs, float), np.array(offsets, int)
 
 

@@ -1,0 +1,79 @@
+"""Parity at the benchmark's own shapes (run on an MI355X): 256 real FP64
+bands on the 112^3 grid with 64 atoms x 8 projectors, built by
+bench.build_workload -- the launch shapes the benchmark times (LFC add band
+tiles of 32, integrate tiles of 16 bands, the LDS-ring stencil with band
+groups and the XCD-ordered block map, the V-cycle at 256 bands) compared
+with the oracle on sampled bands: H psi (gpaw/hamiltonian.py:445-479), the
+residual (H - eps S) psi (eigensolver.py:153-172) and the preconditioner
+(preconditioner.py:35-93)."""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip('torch')
+
+pytestmark = pytest.mark.gpu
+
+if not torch.cuda.is_available():
+    pytest.skip('no GPU', allow_module_level=True)
+
+from oracle import grid as og  # noqa: E402
+from oracle import lfc as olfc  # noqa: E402
+from oracle import precond as opc  # noqa: E402
+from oracle import rmmdiis as orm  # noqa: E402
+from oracle import spline as osp  # noqa: E402
+from oracle import stencil as ost  # noqa: E402
+
+SAMPLES = [0, 131, 255]
+
+
+def relerr(a, b):
+    return abs(a - b).max() / abs(b).max()
+
+
+@pytest.fixture(scope='module')
+def workload():
+    import bench
+    w, gd, pt, vt, kin, H, psi, out = bench.build_workload(112, 64, 256)
+    ogd = og.Grid((112,) * 3, (w.L,) * 3)
+    opt = olfc.LFC(ogd, [[osp.Spline(l, rc, f) for l, rc, f in p]
+                         for p in w.proj])
+    opt.set_positions(w.spos_ac)
+    c, o = ost.laplace_stencil(ogd.h_cv, -0.5, 3)
+    oham = orm.Hamiltonian(ogd, (c, o), vt, opt, w.dH)
+    yield w, gd, pt, kin, H, psi, out, oham, ogd
+    del psi, out
+    torch.cuda.empty_cache()
+
+
+def test_full_hpsi_256_bands(workload):
+    w, gd, pt, kin, H, psi, out, oham, ogd = workload
+    assert pt.Mmax == 512
+    P = H.apply(psi, out)
+    torch.cuda.synchronize()
+    x = psi[SAMPLES].cpu().numpy()
+    ref_P = oham.projections(x)
+    assert relerr(P[SAMPLES].cpu().numpy(), ref_P) < 1e-12
+    ref = oham.apply(x)
+    assert relerr(out[SAMPLES].cpu().numpy(), ref) < 1e-12
+
+
+def test_residual_256_bands(workload):
+    w, gd, pt, kin, H, psi, out, oham, ogd = workload
+    eps = torch.linspace(-1.5, 0.5, 256, dtype=torch.float64, device='cuda')
+    H.apply_local_shifted(psi, out, eps)
+    P = pt.integrate_M(psi)
+    pt.add_M(out, H.dH(P, eps_x=eps))
+    torch.cuda.synchronize()
+    x = psi[SAMPLES].cpu().numpy()
+    ref = oham.residual(x, eps[SAMPLES].cpu().numpy())
+    assert relerr(out[SAMPLES].cpu().numpy(), ref) < 1e-12
+
+
+def test_vcycle_256_bands(workload):
+    from gpaw_amd.preconditioner import Preconditioner
+    w, gd, pt, kin, H, psi, out, oham, ogd = workload
+    d = Preconditioner(gd, kin)(psi, out=out)
+    torch.cuda.synchronize()
+    c, o = ost.laplace_stencil(ogd.h_cv, -0.5, 3)
+    ref = opc.Preconditioner(ogd, c, o)(psi[SAMPLES].cpu().numpy())
+    assert relerr(d[SAMPLES].cpu().numpy(), ref) < 1e-12
