@@ -133,69 +133,126 @@ def cpu_baseline(w, gd_args, vt, nb_sample):
                        '(%.2f s each)' % (nb_sample, len(times), t))}
 
 
-def rmmdiis_leg(w, gd, pt, kin, H, nb, sweeps, nconv, tol=1e-8,
-                precision=None, dense=True, solver='rmm-diis'):
-    """RMM-DIIS (niter=3, all bands in one block) or Davidson (niter=2)
-    with the multigrid preconditioner (FP64, or FP32 with
-    precision='single') on the same Hamiltonian, from the atom-centred
-    start.  Error as the reference's (sum_n w_n int |R_n|^2 dv) with w_n = 1
-    for the lowest `nconv` bands, divided by nconv.  Returns timing and
-    convergence ('sweeps' counts Eigensolver.iterate calls)."""
+def eigensolver_run(w, gd, vt, nb, nconv, psi0, sweeps, dtype=np.float64,
+                    k_c=None, solver='rmm-diis', tol=1e-8):
+    """RMM-DIIS (niter=3, all bands in one block) or Davidson (niter=2) with
+    the multigrid preconditioner, wave functions and grid work in `dtype`
+    (float64 / complex128: the plain FP64 form; float32 / complex64: the
+    mixed form, whose subspace matrices are FP32 products with FP64 sums and
+    whose eigh, Cholesky and reductions are FP64), from the start psi0
+    (converted to dtype).  Error as the reference's (sum_n w_n int |R_n|^2
+    dv) with w_n = 1 for the lowest `nconv` bands, divided by nconv; stops
+    at err <= tol.  The timed region runs from the first orthonormalisation
+    to the sweep that reaches tol."""
     import torch
     from gpaw_amd import synthetic as syn
+    from gpaw_amd._arrays import torch_dtype
     from gpaw_amd.eigensolvers import Davidson, RMMDIIS, FDWaveFunctions
-    from gpaw_amd.preconditioner import Preconditioner
-    psi0 = syn.initial_guess(gd, w.pos_av, nb)
-    wfs = FDWaveFunctions(gd, psi0, pt,
-                          Preconditioner(gd, kin, precision=precision))
+    sysm = syn.System(w, gd, vt, dtype, k_c)
+    wfs = FDWaveFunctions(gd, psi0.to(torch_dtype(dtype)).contiguous(),
+                          sysm.pt, sysm.pc, sysm.phase_cd)
     es = RMMDIIS() if solver == 'rmm-diis' else Davidson()
     weights = torch.zeros(nb, dtype=torch.float64, device='cuda')
     weights[:nconv] = 1.0
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    wfs.orthonormalize(H)
+    wfs.orthonormalize(sysm.H)
     errs, t_tol = [], None
     for it in range(sweeps):
-        err = es.iterate(H, wfs, weights) / nconv
+        err = es.iterate(sysm.H, wfs, weights) / nconv
         errs.append(err)
-        if t_tol is None and err <= tol:
+        if err <= tol:
             torch.cuda.synchronize()
             t_tol = time.perf_counter() - t0
             break
     torch.cuda.synchronize()
     t = time.perf_counter() - t0
-    res = {'solver': repr(es) if solver != 'rmm-diis' else
-           'RMMDIIS(niter=3, all bands in one block)',
-           'precond': 'multigrid V-cycle, %s' % (
-               'FP32 (residuals, subspace and Cholesky FP64)'
-               if precision == 'single' else 'FP64'),
-           'start': 'atom-centred s/p Gaussians, orthonormalised',
+    res = {'solver': 'RMMDIIS(niter=3, all bands in one block)'
+           if solver == 'rmm-diis' else repr(es),
+           'dtype': str(np.dtype(dtype)),
+           'precond': 'multigrid V-cycle (%s)' % np.dtype(dtype),
            'bands': nb, 'bands_converged': nconv,
            'sweeps': len(errs), 'seconds': round(t, 3),
            'ms_per_sweep': round(1e3 * t / max(len(errs), 1), 2),
-           'err_first': errs[0] if errs else None,
-           'err_last': errs[-1] if errs else None,
+           'errors': [float('%.3e' % e) for e in errs],
            'seconds_to_1e-8': None if t_tol is None else round(t_tol, 3),
            'eps_lowest': float(wfs.eps[0])}
-    if not dense:
-        return res
-    # the dense contractions of a sweep, timed alone (rocBLAS FP64 MFMA)
-    a = wfs.psit.reshape(nb, -1)
-    b = torch.empty_like(wfs.psit).reshape(nb, -1)
-    b.copy_(a)
-    U = torch.linalg.qr(torch.randn((nb, nb), dtype=torch.float64,
-                                    device='cuda'))[0]
-    t_h = _time_ms(lambda: a @ b.T)
-    t_r = _time_ms(lambda: U @ a)
-    flop = 2.0 * nb * nb * a.shape[1]
-    dense_ = {'H_matrix_ms': round(t_h, 3),
-              'H_matrix_TFLOPs': round(flop / t_h / 1e9, 1),
-              'rotation_ms': round(t_r, 3),
-              'rotation_TFLOPs': round(flop / t_r / 1e9, 1),
-              'peak_fp64_mfma_TFLOPs': 78.6}
-    del b
-    res['dense_fp64'] = dense_
-    return res
+    eps = wfs.eps[:nconv].cpu().numpy()
+    del wfs, sysm, es
+    torch.cuda.empty_cache()
+    return res, eps
+
+
+def eigensolver_leg(w, gd, vt, H, nb, nconv, sweeps, k_c=None,
+                    davidson=False):
+    """The second metric's plain-FP64-multigrid and mixed-precision RMM-DIIS
+    runs (and optionally Davidson) from one LCAO-like start (Rayleigh-Ritz
+    in atom-centred functions, as an SCF starts): time to err <= 1e-8 and
+    the mixed form's eigenvalue error against the FP64 run (the FP64 path
+    is pinned to the oracle by the GPU parity tests)."""
+    import torch
+    from gpaw_amd import synthetic as syn
+    cplx = k_c is not None
+    d64, d32 = (np.complex128, np.complex64) if cplx else (np.float64,
+                                                           np.float32)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    if cplx:
+        Hk = syn.System(w, gd, vt, d64, k_c).H
+        psi0 = syn.lcao_start(gd, w, Hk, nb, dtype=complex, k_c=k_c)
+        del Hk
+    else:
+        psi0 = syn.lcao_start(gd, w, H, nb)
+    torch.cuda.synchronize()
+    t_start = time.perf_counter() - t0
+    out = {'start': 'LCAO-like: lowest Rayleigh-Ritz vectors in 12 '
+                    'atom-centred functions per atom (projector shapes + s/p '
+                    'Gaussians), %.2f s, not in the timed region' % t_start,
+           'k_c': list(k_c) if cplx else [0, 0, 0]}
+    r64, e64 = eigensolver_run(w, gd, vt, nb, nconv, psi0, sweeps, d64, k_c)
+    r32, e32 = eigensolver_run(w, gd, vt, nb, nconv, psi0, sweeps, d32, k_c)
+    r32['eigenvalue_max_abs_err_vs_fp64_Ha'] = float(abs(e32 - e64).max())
+    out['fp64_multigrid'] = r64
+    out['mixed_fp32'] = r32
+    if r64['seconds_to_1e-8'] and r32['seconds_to_1e-8']:
+        out['speedup_mixed_vs_fp64'] = round(
+            r64['seconds_to_1e-8'] / r32['seconds_to_1e-8'], 2)
+    if davidson:
+        rd, ed = eigensolver_run(w, gd, vt, nb, nconv, psi0, sweeps, d64,
+                                 k_c, solver='davidson')
+        rd['eigenvalue_max_abs_diff_vs_rmmdiis_Ha'] = float(abs(ed - e64).max())
+        out['davidson_fp64'] = rd
+    del psi0
+    torch.cuda.empty_cache()
+    return out
+
+
+def dense_leg(nb, N):
+    """The subspace contractions of a sweep at the workload shape, timed
+    alone: gh_gram (lower triangle on the matrix cores) against torch's
+    full product, and the rotation U psi (rocBLAS), per precision."""
+    import torch
+    from gpaw_amd import blas
+    out = {'peak_TFLOPs': {'fp64_mfma': 78.6, 'fp32_mfma': 157.3}}
+    for name, dt in (('f64', torch.float64), ('f32', torch.float32),
+                     ('c64', torch.complex64), ('c128', torch.complex128)):
+        a = torch.randn((nb, N), dtype=dt, device='cuda')
+        U = torch.randn((nb, nb), dtype=dt, device='cuda')
+        cf = 8 if a.is_complex() else 2
+        flop = cf * nb * nb * N
+        t_g = _time_ms(lambda: blas.gram(a, hermitian=True))
+        t_r = _time_ms(lambda: U @ a)
+        rec = {'gram_lower_ms': round(t_g, 3),
+               'gram_TFLOPs_executed': round(0.75 * flop / t_g / 1e9, 1),
+               'rotation_ms': round(t_r, 3),
+               'rotation_TFLOPs': round(flop / t_r / 1e9, 1)}
+        if not a.is_complex():
+            t_t = _time_ms(lambda: a @ a.T)
+            rec['torch_full_product_ms'] = round(t_t, 3)
+        out[name] = rec
+        del a
+        torch.cuda.empty_cache()
+    return out
 
 
 def _time_ms(fn, reps=5):
@@ -420,18 +477,13 @@ def main():
     rmm = None
     if args.rmm_sweeps > 0:
         out = None
-        rmm = rmmdiis_leg(w, gd, pt, kin, H, nb, args.rmm_sweeps,
-                          min(args.rmm_converge, nb))
-        rmm['single_precision_vcycle'] = rmmdiis_leg(
-            w, gd, pt, kin, H, nb, args.rmm_sweeps,
-            min(args.rmm_converge, nb), precision='single', dense=False)
-
-    dav = None
-    if args.dav_iters > 0:
-        out = None
-        dav = rmmdiis_leg(w, gd, pt, kin, H, nb, args.dav_iters,
-                          min(args.rmm_converge, nb), dense=False,
-                          solver='davidson')
+        rmm = eigensolver_leg(w, gd, vt, H, nb, min(args.rmm_converge, nb),
+                              args.rmm_sweeps, davidson=args.dav_iters > 0)
+        if not args.no_c3:
+            rmm['c3_complex_k'] = eigensolver_leg(
+                w, gd, vt, H, nb, min(args.rmm_converge, nb),
+                args.rmm_sweeps, k_c=(0.25, 0.0, 0.0))
+        rmm['dense'] = dense_leg(nb, ngpts)
 
     cpu = None
     if not args.no_cpu:
@@ -465,7 +517,6 @@ def main():
                                '(T psi + V psi, one pass)',
                      'kernel_ms': round(stencil_ms, 4)},
         'rmmdiis': rmm,
-        'davidson': dav,
         'c3_single_precision': c3,
         'transformers': trans,
         'density_and_poisson': scf_next,

@@ -58,6 +58,9 @@ _SIGNATURES = {
                                  _c.POINTER(P), D, P, P, P]),
     'gh_multi_lincomb': (_c.c_int, [_c.c_int, I64, I64, _c.c_int,
                                     _c.POINTER(P), P, P, P]),
+    'gh_gram_work_bytes': (I64, [_c.c_int, I64, I64, I64, _c.c_int]),
+    'gh_gram': (_c.c_int, [_c.c_int, I64, I64, I64, P, I64, P, I64, D,
+                           _c.c_int, _c.c_int, P, P, I64, P]),
     'gh_spline_coefs': (_c.c_int, [_c.c_int, D, PD, PD]),
     'gh_spline_to_grid': (I64, [_c.c_int, _c.c_int, D, PD, PI64, PI64, PD,
                                 PD, PI64, PI64, PD, PI32]),

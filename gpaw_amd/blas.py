@@ -99,3 +99,51 @@ def multi_lincomb(vecs, coef, out=None):
         dtype_code(v0.dtype), nb, n, k, ptrs, c.data_ptr(), out.data_ptr(),
         stream_ptr()))
     return out
+
+
+def gram(a_xG, b_yG=None, alpha=1.0, hermitian=False, conj=False, out=None):
+    """C[x, y] = alpha * sum_G a[x, G] conj(b[y, G]) on the matrix cores
+    (gh_gram): float64 result for real data, complex128 for complex data,
+    whatever the input precision (FP32 / complex64 products are summed in
+    FP32 within k-chunks and the chunks in FP64).  b_yG=None means a_xG.
+    hermitian: the product is known to be Hermitian (m == n); only the lower
+    triangle is formed from the data and the upper one is its conjugate.
+    conj: return the conjugate, sum conj(a) b."""
+    if b_yG is None:
+        b_yG = a_xG
+    assert a_xG.dtype == b_yG.dtype and a_xG.is_cuda and b_yG.is_cuda
+    a = a_xG.reshape(a_xG.shape[0], -1) if a_xG.dim() > 1 else a_xG[None]
+    b = b_yG.reshape(b_yG.shape[0], -1) if b_yG.dim() > 1 else b_yG[None]
+    assert a.shape[1] == b.shape[1], (a.shape, b.shape)
+    assert a.stride(1) == 1 and b.stride(1) == 1
+    m, n, k = a.shape[0], b.shape[0], a.shape[1]
+    if hermitian and m != n:
+        raise ValueError('hermitian product needs equal row counts')
+    cplx = a.is_complex()
+    shape = (m, n)
+    if out is None:
+        out = torch.empty(shape, dtype=torch.complex128 if cplx else
+                          torch.float64, device='cuda')
+    assert tuple(out.shape) == shape and out.is_contiguous()
+    assert out.dtype == (torch.complex128 if cplx else torch.float64)
+    es = a.element_size()
+    if ((k * es) % 16 or (a.stride(0) * es) % 16 or (b.stride(0) * es) % 16
+            or a.data_ptr() % 16 or b.data_ptr() % 16):
+        # rows the 16-byte matrix-core loader cannot take (odd grid sizes,
+        # misaligned views): FP64 GEMM on the GPU
+        mdt = out.dtype
+        c = alpha * (a.to(mdt) @ b.to(mdt).conj().T)
+        if hermitian:
+            low = torch.tril(c, -1)
+            c = low + low.conj().T + torch.diag(torch.diagonal(c).real).to(mdt)
+        out.copy_(c.conj() if conj else c)
+        return out
+    L = _lib.lib()
+    code = dtype_code(a.dtype)
+    nwork = int(L.gh_gram_work_bytes(code, m, n, k, int(hermitian)))
+    work = torch.empty(max(nwork, 1), dtype=torch.uint8, device='cuda')
+    _lib.check(L.gh_gram(
+        code, m, n, k, a.data_ptr(), a.stride(0), b.data_ptr(), b.stride(0),
+        float(alpha), int(hermitian), int(conj), out.data_ptr(),
+        work.data_ptr(), nwork, stream_ptr()))
+    return out

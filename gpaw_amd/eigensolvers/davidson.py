@@ -20,9 +20,10 @@ one block:
      expansion (davidson.py:126-127).  No orthonormalisation afterwards: the
      eigenvectors of the pair are S-orthonormal.
 
-The four 256 x 1.4 M products per expansion and the rotation are FP64 GEMMs
-on the GPU (torch / rocBLAS); grid work (stencil, projectors, V-cycle) runs
-in libgpawhip.
+The four 256 x 1.4 M products per expansion are gh_gram calls (matrix
+cores, FP64 sums; the diagonal blocks from lower triangles) and the rotation
+is a GEMM through torch (rocBLAS) in the data precision; grid work
+(stencil, projectors, V-cycle) runs in libgpawhip.
 """
 import torch
 
@@ -91,16 +92,18 @@ class Davidson(Eigensolver):
         S = torch.zeros_like(H)
         H[:nb, :nb] = torch.diag(wfs.eps).to(mdt)
         S[:nb, :nb] = torch.eye(nb, dtype=mdt, device='cuda')
-        H22 = dv * (psi2.conj() @ h2.T)
-        H21 = dv * (h2.conj() @ psi.T)
-        S22 = dv * (psi2.conj() @ psi2.T)
-        S21 = dv * (psi2.conj() @ psi.T)
+        # X_xy = dv sum conj(a_x) b_y on the matrix cores (FP64 sums); the
+        # diagonal blocks from their lower triangles
+        H22 = blas.gram(psi2, h2, alpha=dv, hermitian=True, conj=True)
+        H21 = blas.gram(h2, psi, alpha=dv, conj=True)
+        S22 = blas.gram(psi2, alpha=dv, hermitian=True, conj=True)
+        S21 = blas.gram(psi2, psi, alpha=dv, conj=True)
         if P2 is not None:
-            H22 = H22 + P2.conj() @ ham.dH(P2).T
-            H21 = H21 + P2.conj() @ ham.dH(P).T
+            H22 = H22 + (P2.conj() @ ham.dH(P2).T).to(mdt)
+            H21 = H21 + (P2.conj() @ ham.dH(P).T).to(mdt)
             if ham.dS_flat is not None:
-                S22 = S22 + P2.conj() @ ham.dS(P2).T
-                S21 = S21 + P2.conj() @ ham.dS(P).T
+                S22 = S22 + (P2.conj() @ ham.dS(P2).T).to(mdt)
+                S21 = S21 + (P2.conj() @ ham.dS(P).T).to(mdt)
         H[nb:, nb:] = H22
         H[nb:, :nb] = H21
         S[nb:, nb:] = S22
@@ -125,8 +128,8 @@ class Davidson(Eigensolver):
             self.apply_h_local(ham, wfs, psit2, R)
             H, S = self._pair(ham, wfs, psit, P, psit2, P2, R)
             e, C = eigh_pair_lower(H, S)
-            C1 = C[:nb, :nb].T.contiguous()
-            C2 = C[nb:, :nb].T.contiguous()
+            C1 = C[:nb, :nb].T.contiguous().to(psit.dtype)
+            C2 = C[nb:, :nb].T.contiguous().to(psit.dtype)
             new = C1 @ wfs.flat(psit)
             new.addmm_(C2, wfs.flat(psit2))
             wfs.psit = new.reshape(psit.shape)

@@ -56,13 +56,15 @@ class Eigensolver:
         self.apply_h_local(ham, wfs, psit, Htpsit)
         psi = wfs.flat(psit)
         hpsi = wfs.flat(Htpsit)
-        H = wfs.dv * (psi.conj() @ hpsi.T)
+        # H_xy = dv <psi_x|T + v|psi_y>: lower triangle on the matrix cores
+        # (FP64 sums whatever the data precision), the upper one mirrored
+        H = blas.gram(psi, hpsi, alpha=wfs.dv, hermitian=True, conj=True)
         P = wfs.projections()
         if P is not None:
-            H = H + P.conj() @ ham.dH(P).T
-        H = 0.5 * (H + H.conj().T)
+            H = H + (P.conj() @ ham.dH(P).T).to(H.dtype)
+            H = 0.5 * (H + H.conj().T)
         eps, C = torch.linalg.eigh(H)
-        U = C.T.contiguous()
+        U = C.T.contiguous().to(psi.dtype)   # rotations in the data precision
         wfs.psit = (U @ psi).reshape(psit.shape)
         Htpsit = (U @ hpsi).reshape(psit.shape)
         if P is not None:

@@ -20,9 +20,14 @@ RMMDIIS.iterate restates Eigensolver.iterate + RMMDIIS.iterate_one_k_point
   4. orthonormalisation (fdpw.py:394-424): S = dv psi psi^H + P dS P^H =
      L L^H, psi <- L^-1 psi, P <- L^-1 P.
 
-Grid work (stencils, projectors, V-cycle, band-wise dots and axpys) runs in
-libgpawhip; the nbands x nbands algebra (S, H, eigh, Cholesky, rotations)
-runs on the GPU in FP64 through torch (rocBLAS / rocSOLVER).
+Grid work (stencils, projectors, V-cycle, band-wise dots and axpys) and the
+subspace matrices S and H (gh_gram: lower triangles on the matrix cores,
+FP64 sums) run in libgpawhip; eigh and Cholesky run in FP64 on the GPU
+through torch (rocSOLVER) and the rotations psi <- U psi through torch
+(rocBLAS) in the precision of the wave functions.  With float32 /
+complex64 wave functions (the mixed-precision form) everything on the grid
+is single precision while S, H, eigh, Cholesky and every reduction (errors,
+DIIS matrices) stay FP64.
 """
 import torch
 
@@ -60,10 +65,11 @@ class FDWaveFunctions:
         """S' = dv psi psi^H + P dS P^H (the complex conjugate of
         S_nm = <psi_n|S|psi_m>; equal to it for real wave functions)."""
         psi = self.flat(self.psit)
-        S = self.dv * (psi @ psi.conj().T)
+        S = blas.gram(psi, alpha=self.dv, hermitian=True)
         P = self.projections()
         if P is not None and ham is not None and ham.dS_flat is not None:
-            S = S + ham.dS(P) @ P.conj().T
+            S = S + (ham.dS(P) @ P.conj().T).to(S.dtype)
+            S = 0.5 * (S + S.conj().T)
         return S
 
     def add_to_density_from_k_point_with_occupation(self, nt_G, f_n):
@@ -78,6 +84,7 @@ class FDWaveFunctions:
         Linv = torch.linalg.solve_triangular(
             L, torch.eye(len(S), dtype=S.dtype, device=S.device), upper=False)
         psi = self.flat(self.psit)
+        Linv = Linv.to(psi.dtype)   # the rotation in the data precision
         # the rotated wave functions go to a new buffer (no copy back):
         # psit is re-bound, as is every rotation below
         self.psit = (Linv @ psi).reshape(self.psit.shape)

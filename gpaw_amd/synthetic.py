@@ -163,6 +163,7 @@ def lcao_start(gd, w, H, nbands, dtype=float, k_c=None, basis=None):
     harmless.  Returns device wave functions (nbands, n0, n1, n2),
     orthonormal (dv sum psi_n^* psi_m = delta_nm)."""
     import torch
+    from . import blas
     from .lfc import LFC
     from .spline import Spline
     from ._arrays import torch_dtype
@@ -182,16 +183,39 @@ def lcao_start(gd, w, H, nbands, dtype=float, k_c=None, basis=None):
     ph = None if k_c is None else gd.phase_cd(k_c)
     H.apply(phi, hphi, ph)
     f = phi.reshape(nbasis, -1)
-    S = gd.dv * (f.conj() @ f.T)
+    S = blas.gram(f, alpha=gd.dv, hermitian=True, conj=True)
     if H.pt is not None and H.dS_flat is not None:
         P = H.pt.integrate_M(phi)
         S = S + P.conj() @ H.dS(P).T
-    Hm = gd.dv * (f.conj() @ hphi.reshape(nbasis, -1).T)
+    Hm = blas.gram(f, hphi.reshape(nbasis, -1), alpha=gd.dv, hermitian=True,
+                   conj=True)
     del hphi
-    Hm = 0.5 * (Hm + Hm.conj().T)
     s, U = torch.linalg.eigh(0.5 * (S + S.conj().T))
     keep = s > 1e-9 * s.max()
     X = U[:, keep] / torch.sqrt(s[keep])          # X^H S X = 1
     e, V = torch.linalg.eigh(X.conj().T @ Hm @ X)
     C = X @ V[:, :nbands]                          # basis -> band
     return (C.T @ f).reshape((nbands,) + tuple(gd.n_c)).contiguous()
+
+
+class System:
+    """The FD Hamiltonian of a workload in one precision: LFC of the
+    projectors, 19-point kinetic stencil, FDHamiltonian (potential and dH in
+    the data's real precision / FP64) and the multigrid preconditioner."""
+
+    def __init__(self, w, gd, vt, dtype=float, k_c=None):
+        from .fd_operators import Laplace
+        from .hamiltonian import FDHamiltonian
+        from .lfc import LFC
+        from .preconditioner import Preconditioner
+        from .spline import Spline
+        self.dtype = dtype
+        self.k_c = k_c
+        kd = None if k_c is None else np.array([k_c], float)
+        self.pt = LFC(gd, [[Spline.from_data(l, rc, f) for l, rc, f in p]
+                           for p in w.proj], kd, dtype=dtype)
+        self.pt.set_positions(w.spos_ac)
+        self.kin = Laplace(gd, -0.5, 3, dtype)
+        self.H = FDHamiltonian(gd, self.kin, vt, self.pt, w.dH, dtype=dtype)
+        self.pc = Preconditioner(gd, self.kin, dtype)
+        self.phase_cd = None if k_c is None else gd.phase_cd(k_c)

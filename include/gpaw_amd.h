@@ -259,6 +259,28 @@ int gh_multi_lincomb(int dtype, int64_t nbands, int64_t n, int k,
 int gh_add_to_density(int dtype, int64_t nbands, int64_t n, const double* f_dev,
                       const void* psit_dev, double* nt_dev, void* stream);
 
+/* ------------------------------------------------- band overlap matrices --
+ * Replaces the rk / r2k / mmm(..., 'N', ..., 'C') contractions of
+ * gpaw/utilities/blas.py:39,273,314 (c/blas.c:48,101,140) as used for the
+ * subspace matrices (gpaw/matrix.py:229-327, wavefunctions/arrays.py:262-340,
+ * GridDescriptor.integrate grid_descriptor.py:275-326):
+ *   C[i][j] = alpha * sum_g A[i][g] * conj(B[j][g])   (conj_out: conj of it)
+ * A_dev: m rows, B_dev: n rows, each of k elements of dtype with row pitch
+ * lda / ldb elements (k * element size, lda and ldb must be multiples of 16
+ * bytes; A and B 16-byte aligned).  C_dev: m x n row-major float64 (real
+ * dtypes) or complex128 (complex dtypes).  hermitian = 1 (m == n): only the
+ * lower triangle is formed from the data and the upper one is its conjugate
+ * (for products known to be Hermitian, like rk's lower triangle).
+ * FP32/complex64 on v_mfma_f32_32x32x2f32 and FP64/complex128 on
+ * v_mfma_f64_16x16x4f64; products are summed in the element precision within
+ * k-chunks and the chunks in FP64 (fixed order: deterministic).
+ * work_dev: device scratch of gh_gram_work_bytes(...) bytes. */
+int64_t gh_gram_work_bytes(int dtype, int64_t m, int64_t n, int64_t k, int hermitian);
+int gh_gram(int dtype, int64_t m, int64_t n, int64_t k, const void* A_dev,
+            int64_t lda, const void* B_dev, int64_t ldb, double alpha,
+            int hermitian, int conj_out, double* C_dev, void* work_dev,
+            int64_t work_bytes, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -29,10 +29,15 @@ from test_oracle_rmmdiis import dense_pair, small_system, start_guess  # noqa
 CASES = [None, (0.25, -0.1, 0.3)]
 
 
-def gpu_system(ham_o):
+def gpu_system(ham_o, single=False):
+    """The oracle's system on the GPU; single=True: float32 / complex64 wave
+    functions, stencil, projectors and V-cycle (the mixed-precision form;
+    dH, dS and the subspace algebra stay FP64)."""
     raw = ham_o.raw
     N, h, k_c = raw['N'], raw['h'], raw['k_c']
     dtype = float if k_c is None else complex
+    if single:
+        dtype = np.float32 if k_c is None else np.complex64
     gd = GridDescriptor((N,) * 3, (N * h,) * 3)
     spl = [[Spline.from_data(l, raw['rcut'], f) for l, f in fs]
            for fs in raw['splines']]
@@ -242,3 +247,36 @@ def test_rmmdiis_start_dependence():
     errors = [es.iterate(ham, wfs, weights) for _ in range(25)]
     assert errors[-1] < 1e-10
     assert abs(host(wfs.eps)[:nconv] - w[:nconv]).max() < 1e-9
+
+
+@pytest.mark.parametrize('k_c', CASES)
+def test_rmmdiis_single_precision_converges(k_c):
+    """Mixed precision: float32 / complex64 wave functions and grid work,
+    subspace matrices from FP32 products with FP64 sums (gh_gram), FP64
+    eigh / Cholesky and reductions.  From the same start as the FP64 run it
+    reaches an error below 1e-9 and the dense FP64 eigenvalues to 1e-6 Ha
+    (the target's bound), tracking the FP64 run's errors on the way."""
+    ham_o, _, rng = small_system(N=16, h=0.4, k_c=k_c, seed=4)
+    Hm, Sm = dense_pair(ham_o)
+    nb, nconv = 8, 6
+    w, W = sl.eigh(Hm, Sm, subset_by_index=[0, nb - 1])
+    psi = start_guess(ham_o, rng, nb, W, 0.3)
+    weights = [1.0] * nconv + [0.0] * (nb - nconv)
+    runs = []
+    for single in (False, True):
+        gd, ham, pc = gpu_system(ham_o, single)
+        start = dev(psi).to(torch.float32 if k_c is None and single else
+                            torch.complex64 if single else dev(psi).dtype)
+        wfs = FDWaveFunctions(gd, start, ham.pt, pc, ham_o.phase_cd)
+        es = RMMDIIS()
+        errs = [es.iterate(ham, wfs, weights) for _ in range(25)]
+        runs.append((errs, host(wfs.eps)))
+        if single:
+            assert wfs.psit.dtype == start.dtype
+    (e64, eps64), (e32, eps32) = runs
+    assert e32[-1] < 1e-9
+    assert abs(eps32[:nconv] - w[:nconv]).max() < 1e-6
+    assert abs(eps64[:nconv] - w[:nconv]).max() < 1e-10
+    # the single-precision run follows the FP64 one until its floor
+    for a, b in zip(e64[:4], e32[:4]):
+        assert abs(a - b) < 1e-3 * a
