@@ -202,13 +202,29 @@ lfc_add_kernel(const V* __restrict__ c, V* __restrict__ a,
     }
 }
 
-// out[x, M0+i] = sum_j P[x, M0+j] dH[j][i]  (- eps[x] sum_j P dS[j][i])
-template <typename V, typename S>
-__global__ void lfc_dH_kernel(const V* __restrict__ P, const S* __restrict__ dH,
+// FP64 value of a stored coefficient (single-precision data are widened)
+__device__ __forceinline__ double widen(double a) { return a; }
+__device__ __forceinline__ double widen(float a) { return (double)a; }
+template <typename S>
+__device__ __forceinline__ cplx<double> widen(cplx<S> a) {
+    return {(double)a.re, (double)a.im};
+}
+__device__ __forceinline__ void narrow(double a, double* o) { *o = a; }
+__device__ __forceinline__ void narrow(double a, float* o) { *o = (float)a; }
+template <typename S>
+__device__ __forceinline__ void narrow(cplx<double> a, cplx<S>* o) {
+    *o = {(S)a.re, (S)a.im};
+}
+
+// out[x, M0+i] = sum_j P[x, M0+j] dH[j][i]  (- eps[x] sum_j P dS[j][i]),
+// summed in FP64 for every storage type V (W: its FP64 counterpart)
+template <typename V, typename W>
+__global__ void lfc_dH_kernel(const V* __restrict__ P, const double* __restrict__ dH,
                               const int64_t* __restrict__ dH_off,
-                              const S* __restrict__ eps, const S* __restrict__ dS,
+                              const double* __restrict__ eps, const double* __restrict__ dS,
                               const int32_t* __restrict__ M_ptr, V* __restrict__ out,
                               int natoms, int Mmax, int nbands) {
+    using S = double;
     const int x = blockIdx.y;
     const int M = blockIdx.x * blockDim.x + threadIdx.x;
     if (x >= nbands || M >= Mmax) return;
@@ -223,15 +239,15 @@ __global__ void lfc_dH_kernel(const V* __restrict__ P, const S* __restrict__ dH,
     const int i = M - M0;
     const S* H = dH + dH_off[lo];
     const V* Px = P + (size_t)x * Mmax + M0;
-    V acc = vzero<V>();
-    for (int j = 0; j < nm; j++) acc = fma_rs(H[j * nm + i], Px[j], acc);
+    W acc = vzero<W>();
+    for (int j = 0; j < nm; j++) acc = fma_rs(H[j * nm + i], widen(Px[j]), acc);
     if (eps && dS) {
         const S* D = dS + dH_off[lo];
-        V s = vzero<V>();
-        for (int j = 0; j < nm; j++) s = fma_rs(D[j * nm + i], Px[j], s);
+        W s = vzero<W>();
+        for (int j = 0; j < nm; j++) s = fma_rs(D[j * nm + i], widen(Px[j]), s);
         acc = fma_rs(-eps[x], s, acc);
     }
-    out[(size_t)x * Mmax + M] = acc;
+    narrow(acc, out + (size_t)x * Mmax + M);
 }
 
 }  // namespace gh
@@ -553,20 +569,31 @@ int gh_lfc_apply_dH(const gh_lfc* h, const void* P_dev, const double* dH_dev,
                     const double* eps_dev, const double* dS_dev,
                     void* out_dev, int64_t nbands, void* stream) {
     GH_REQUIRE(h && P_dev && dH_dev && out_dev, GH_ERR_ARG, "lfc_apply_dH: null argument");
-    GH_REQUIRE(!single(h->dtype), GH_ERR_UNSUPPORTED,
-               "lfc_apply_dH: single-precision coefficients are converted by the caller");
     GH_REQUIRE(P_dev != out_dev, GH_ERR_ARG, "lfc_apply_dH: P and out must differ");
     if (nbands == 0 || h->Mmax == 0) return GH_OK;
     hipStream_t st = (hipStream_t)stream;
     dim3 grid((h->Mmax + 127) / 128, (unsigned)nbands);
-    if (h->dtype == GH_F64)
-        hipLaunchKernelGGL((lfc_dH_kernel<double, double>), grid, dim3(128), 0, st,
-                           (const double*)P_dev, dH_dev, h->dH_off, eps_dev, dS_dev,
-                           h->M_ptr, (double*)out_dev, h->natoms, h->Mmax, (int)nbands);
-    else
-        hipLaunchKernelGGL((lfc_dH_kernel<cplx<double>, double>), grid, dim3(128), 0, st,
-                           (const cplx<double>*)P_dev, dH_dev, h->dH_off, eps_dev, dS_dev,
-                           h->M_ptr, (cplx<double>*)out_dev, h->natoms, h->Mmax, (int)nbands);
+#define DH_ARGS(V) (const V*)P_dev, dH_dev, h->dH_off, eps_dev, dS_dev, h->M_ptr, \
+                   (V*)out_dev, h->natoms, h->Mmax, (int)nbands
+    switch (h->dtype) {
+        case GH_F64:
+            hipLaunchKernelGGL((lfc_dH_kernel<double, double>), grid, dim3(128), 0, st,
+                               DH_ARGS(double));
+            break;
+        case GH_C128:
+            hipLaunchKernelGGL((lfc_dH_kernel<cplx<double>, cplx<double>>), grid, dim3(128), 0,
+                               st, DH_ARGS(cplx<double>));
+            break;
+        case GH_F32:
+            hipLaunchKernelGGL((lfc_dH_kernel<float, double>), grid, dim3(128), 0, st,
+                               DH_ARGS(float));
+            break;
+        case GH_C64:
+            hipLaunchKernelGGL((lfc_dH_kernel<cplx<float>, cplx<double>>), grid, dim3(128), 0,
+                               st, DH_ARGS(cplx<float>));
+            break;
+    }
+#undef DH_ARGS
     GH_CHECK_LAUNCH();
     return GH_OK;
 }

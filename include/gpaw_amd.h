@@ -184,8 +184,9 @@ int gh_lfc_add(const gh_lfc* h, const void* c_dev, void* a_dev,
  * out[x, M_a + i] = sum_j P[x, M_a + j] dH_a[j][i] for every atom
  * (hamiltonian.py:23-44, apply_non_local_hamilton).  dH_dev holds the
  * atoms' nm_a x nm_a real matrices concatenated (row-major); P and out have
- * the LFC's dtype.  If eps_dev (real, nbands) and dS_dev are given, also
- * subtracts eps[x] * sum_j P[x, j] dS_a[j][i] (eigensolver.py:161-165). */
+ * the LFC's dtype (single-precision P is summed in FP64 and rounded once).
+ * If eps_dev (real, nbands) and dS_dev are given, also subtracts
+ * eps[x] * sum_j P[x, j] dS_a[j][i] (eigensolver.py:161-165). */
 int gh_lfc_apply_dH(const gh_lfc* h, const void* P_dev, const double* dH_dev,
                     const double* eps_dev, const double* dS_dev,
                     void* out_dev, int64_t nbands, void* stream);

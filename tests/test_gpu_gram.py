@@ -78,13 +78,25 @@ def test_gram_hermitian(dtype, m, k):
 
 
 def test_gram_views_and_errors():
-    """Row pitch from a strided band view; misaligned rows are refused."""
+    """Row pitch from a strided band view; the C ABI refuses rows that are
+    not 16-byte aligned (the Python layer then takes an FP64 GEMM on the
+    GPU, also checked)."""
+    from gpaw_amd import _lib
+    from gpaw_amd._arrays import stream_ptr
     a = torch.randn((40, 4096), dtype=torch.float64, device='cuda')
     v = a[::2, :2048]
     c = blas.gram(v)
     ref = exact(v, v, 1.0, False)
     assert float(((c - ref).abs() / bound(v, v)).max()) < 1e-13
-    with pytest.raises(Exception):
-        blas.gram(a[:, 1:2049])   # 8-byte aligned rows
+    w = a[:, 1:2049]              # rows start 8 bytes off a 16-byte boundary
+    L = _lib.lib()
+    out = torch.empty((40, 40), dtype=torch.float64, device='cuda')
+    rc = L.gh_gram(_lib.GH_F64, 40, 40, 2048, w.data_ptr(), 4096,
+                   w.data_ptr(), 4096, 1.0, 1, 0, out.data_ptr(), None, 0,
+                   stream_ptr())
+    assert rc == -4 and b'aligned' in L.gh_last_error()
+    c = blas.gram(w, hermitian=True)
+    ref = exact(w, w, 1.0, False)
+    assert float(((c - ref).abs() / bound(w, w)).max()) < 1e-13
     with pytest.raises(ValueError):
         blas.gram(a[:10], a[:12], hermitian=True)
