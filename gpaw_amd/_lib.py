@@ -56,6 +56,8 @@ _SIGNATURES = {
     'gh_add_to_density': (_c.c_int, [_c.c_int, I64, I64, P, P, P, P]),
     'gh_multi_gram': (_c.c_int, [_c.c_int, I64, I64, _c.c_int,
                                  _c.POINTER(P), D, P, P, P]),
+    'gh_multi_gram_noalloc': (_c.c_int, [_c.c_int, I64, I64, _c.c_int,
+                                         _c.POINTER(P), D, P, P]),
     'gh_multi_lincomb': (_c.c_int, [_c.c_int, I64, I64, _c.c_int,
                                     _c.POINTER(P), P, P, P]),
     'gh_gram_work_bytes': (I64, [_c.c_int, I64, I64, I64, _c.c_int]),

@@ -20,6 +20,7 @@
 //   9. d0 = s (L0 d0' - R) - d0'               (steps 13-14)
 // Nine kernels in place of the reference's twenty array passes.
 #include <algorithm>
+#include <mutex>
 
 #include "common.h"
 #include "internal.h"
@@ -301,6 +302,33 @@ int gram_k(int k, int64_t nbands, int64_t n, const void* const* vecs, double sca
     return GH_ERR_ARG;
 }
 
+// Fixed scratch for gh_multi_gram_noalloc: one array per GPU, alive until
+// the process ends.  Host code never uses the symbol's name as an address
+// (on the host it is only a stand-in); each device's address comes from
+// hipGetSymbolAddress, taken once per device with that device current.
+constexpr int64_t kNoAllocScratchBytes = 16ll << 20;   // 16 MiB
+__device__ __align__(256) double scratch[kNoAllocScratchBytes / sizeof(double)];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+int scratch_for_current_device(double** out) {
+    int dev = 0;
+    GH_CHECK_HIP(hipGetDevice(&dev));
+    GH_REQUIRE(dev >= 0 && dev < kMaxDevices, GH_ERR_UNSUPPORTED,
+               "multi_gram_noalloc: device %d beyond the %d supported", dev, kMaxDevices);
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        GH_CHECK_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        GH_REQUIRE(p != nullptr, GH_ERR_HIP, "multi_gram_noalloc: no scratch address");
+        g_scratch_ptr[dev] = (double*)p;
+    }
+    *out = g_scratch_ptr[dev];
+    return GH_OK;
+}
+
 // Pseudo-density accumulation (c/utilities.c:313-356, called per band by
 // wavefunctions/fd.py:95-100): nt[G] += sum_n f_n |psi_n(G)|^2.  One thread
 // per grid point sweeps the bands in order (the reference's order of the
@@ -442,6 +470,24 @@ int gh_multi_gram(int dtype, int64_t nbands, int64_t n, int k,
     }
     set_error("multi_gram: bad dtype");
     return GH_ERR_ARG;
+}
+
+int gh_multi_gram_noalloc(int dtype, int64_t nbands, int64_t n, int k,
+                          const void* const* vecs_dev, double scale, double* out_dev,
+                          void* stream) {
+    GH_REQUIRE(k >= 1 && k <= kGramMax && nbands >= 0, GH_ERR_ARG,
+               "multi_gram_noalloc: k = %d out of range 1..%d", k, kGramMax);
+    const int64_t need = gh_multi_gram_work(k, nbands) * (int64_t)sizeof(double);
+    GH_REQUIRE(need <= kNoAllocScratchBytes, GH_ERR_ARG,
+               "multi_gram_noalloc: %lld bands x k = %d need %lld bytes of scratch, "
+               "more than the fixed %lld",
+               (long long)nbands, k, (long long)need, (long long)kNoAllocScratchBytes);
+    double* work = nullptr;
+    int rc = scratch_for_current_device(&work);
+    if (rc != GH_OK) return rc;
+    // the partial kernel writes every slot the final kernel reads, so the
+    // scratch needs no clearing between calls
+    return gh_multi_gram(dtype, nbands, n, k, vecs_dev, scale, out_dev, work, stream);
 }
 
 int gh_add_to_density(int dtype, int64_t nbands, int64_t n, const double* f_dev,

@@ -240,6 +240,19 @@ int gh_multi_gram(int dtype, int64_t nbands, int64_t n, int k,
                   const void* const* vecs_dev, double scale, double* out_dev,
                   double* work_dev, void* stream);
 
+/* gh_multi_gram without a workspace argument and without any allocation:
+ * no counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  The partial sums go to a fixed 16 MiB device array
+ * per GPU (held until the process ends), so gh_multi_gram_work(k, nbands)
+ * doubles must fit in 16 MiB (e.g. k = 6: up to 780 bands; k = 2: 5461);
+ * larger calls return GH_ERR_ARG.  Every call on one GPU shares that array:
+ * calls that overlap in time on one GPU (several streams or threads) race.
+ * Results are identical to gh_multi_gram's. */
+int gh_multi_gram_noalloc(int dtype, int64_t nbands, int64_t n, int k,
+                          const void* const* vecs_dev, double scale,
+                          double* out_dev, void* stream);
+
 /* Per-band linear combination of k vectors, one pass over the data:
  *   out[x, G] = sum_{i<k} c[x, i] v_i[x, G],  1 <= k <= 8,
  * accumulated in the order given.  Replaces the DIIS update of
