@@ -88,6 +88,51 @@ class GridDescriptor:
         return np.dot(np.indices(self.n_c, float).T + self.beg_c,
                       self.h_cv).T.copy()
 
+    def integrate(self, a_xg, b_yg=None, global_integral=True,
+                  hermitian=False):
+        """Integrals over the grid (grid_descriptor.py:275-326).
+
+        One array: a_x = dv sum_g a_xg.  Two: r[x, y] = dv sum_g
+        conj(a_xg) b_yg -- the rk form when b_yg is a_xg, the r2k form
+        (symmetrised) when hermitian, otherwise the mmm form -- on the
+        matrix cores with FP64 sums (gh_gram).  Returns a device tensor, or
+        a Python scalar for a 0-d result."""
+        from .utilities.blas import r2k, rk, mmm
+        a = a_xg if isinstance(a_xg, torch.Tensor) else torch.from_numpy(
+            np.ascontiguousarray(a_xg)).cuda()
+        xshape = tuple(a.shape[:-3])
+        if b_yg is None:
+            result = a.reshape(xshape + (-1,)).sum(dim=-1) * self.dv
+            return result.item() if result.ndim == 0 else result
+        same = b_yg is a_xg
+        b = a if same else (b_yg if isinstance(b_yg, torch.Tensor) else
+                            torch.from_numpy(np.ascontiguousarray(b_yg)).cuda())
+        gsize = int(np.prod(a.shape[-3:]))
+        A = a.reshape((-1, gsize)).contiguous()
+        B = b.reshape((-1, gsize)).contiguous()
+        mdt = torch.complex128 if A.is_complex() else torch.float64
+        if A.dtype != mdt:
+            # single-precision data: FP32 products, FP64 sums (gh_gram)
+            from .blas import gram
+            if same:
+                res = gram(A, alpha=self.dv, hermitian=True)
+            elif hermitian:
+                G = gram(B, A)
+                res = 0.5 * self.dv * (G + G.conj().T)
+            else:
+                res = gram(B, A, alpha=self.dv)
+        else:
+            res = torch.zeros((len(B), len(A)), dtype=mdt, device='cuda')
+            if same:
+                rk(self.dv, A, 0.0, res)
+            elif hermitian:
+                r2k(0.5 * self.dv, A, B, 0.0, res)
+            else:
+                mmm(self.dv, B, 'N', A, 'C', 0.0, res)
+        yshape = tuple(b.shape[:-3])
+        result = res.T.reshape(xshape + yshape)
+        return result.item() if result.ndim == 0 else result
+
     def get_boxes(self, spos_c, rcut, cut=True):
         """Boxes of grid points that may lie within rcut of the scaled
         position spos_c, one box per periodic image the sphere reaches
