@@ -34,6 +34,13 @@ _SIGNATURES = {
     'gh_stencil_apply_shifted': (_c.c_int, [P, P, P, I64, PD, P, P, P]),
     'gh_stencil_set_tile': (_c.c_int, [P, _c.c_int, _c.c_int, _c.c_int]),
     'gh_stencil_relax': (_c.c_int, [P, _c.c_int, P, P, P, _c.c_int, D, P]),
+    'gh_stencil_batch_create': (_c.c_int, [_c.POINTER(P), _c.c_int,
+                                           _c.POINTER(P), PI64, PI32, PI32,
+                                           _c.c_int]),
+    'gh_stencil_batch_destroy': (_c.c_int, [P]),
+    'gh_stencil_batch_info': (_c.c_int, [P, PI64, PI32, PI32]),
+    'gh_stencil_batch_apply': (_c.c_int, [P, _c.POINTER(P), _c.POINTER(P),
+                                          _c.POINTER(P), P]),
     'gh_transformer_create': (_c.c_int, [_c.POINTER(P), PI64, PI64, _c.c_int,
                                          PI64, PI64, PI32, _c.c_int,
                                          _c.c_int]),

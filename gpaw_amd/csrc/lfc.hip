@@ -38,8 +38,29 @@ constexpr int kAddBT = 32;      // bands per add wavefront (at most)
 constexpr int kAddGrp = 32;     // bands loaded together in add
 constexpr int kAddChunk = 64;   // entries per add wavefront (one per lane)
 
+// FP64 counterpart of a storage type: single-precision data are summed in
+// FP64 (the projections are reductions over thousands of sphere points)
+template <typename V> struct Wide { using type = V; };
+template <> struct Wide<float> { using type = double; };
+template <> struct Wide<cplx<float>> { using type = cplx<double>; };
+__device__ __forceinline__ double widen(double a) { return a; }
+__device__ __forceinline__ double widen(float a) { return (double)a; }
+template <typename S>
+__device__ __forceinline__ cplx<double> widen(cplx<S> a) {
+    return {(double)a.re, (double)a.im};
+}
+__device__ __forceinline__ void narrow(double a, double* o) { *o = a; }
+__device__ __forceinline__ void narrow(double a, float* o) { *o = (float)a; }
+template <typename S>
+__device__ __forceinline__ void narrow(cplx<double> a, cplx<S>* o) {
+    *o = {(S)a.re, (S)a.im};
+}
+
 template <typename V> struct IntBT { static constexpr int value = 16; };
 template <> struct IntBT<cplx<double>> { static constexpr int value = 8; };
+// complex64 sums in complex128: 8 bands keep the LDS of T and the FP64
+// reduction buffer at 48 KB
+template <> struct IntBT<cplx<float>> { static constexpr int value = 8; };
 
 // integrate: one 256-thread block per (atom, band tile).  Per chunk of 256
 // entries every thread gathers a[x, G_e] for its entry and the tile's bands
@@ -59,7 +80,11 @@ lfc_integrate_kernel(const V* __restrict__ a, V* __restrict__ c,
                      const S* __restrict__ A, const S* __restrict__ phase_w,
                      int64_t ngpts, int Mmax, int nbands, S dv) {
     constexpr int BT = IntBT<V>::value;
+    using W = typename Wide<V>::type;
+    constexpr bool kWide = sizeof(W) > sizeof(V);
     __shared__ V T[BT][kIntChunk];
+    // reduction buffer: T itself when the sums have the storage type
+    __shared__ W Tw[kWide ? BT * kIntChunk : 1];
     const int atom = blockIdx.x;
     const int x0 = blockIdx.y * BT;
     const int M0 = M_ptr[atom];
@@ -95,9 +120,9 @@ lfc_integrate_kernel(const V* __restrict__ a, V* __restrict__ c,
         }
     };
 
-    V acc[BT];
+    W acc[BT];
 #pragma unroll
-    for (int b = 0; b < BT; b++) acc[b] = vzero<V>();
+    for (int b = 0; b < BT; b++) acc[b] = vzero<W>();
     gather(e0);
     for (int64_t eb = e0; eb < e1; eb += kIntChunk) {
         __syncthreads();
@@ -110,13 +135,14 @@ lfc_integrate_kernel(const V* __restrict__ a, V* __restrict__ c,
             for (int el = grp; el < n; el += ngrp) {
                 const S Am = Aa[(eb - e0 + el) * nm + m];
 #pragma unroll
-                for (int b = 0; b < BT; b++) acc[b] = fma_rs(Am, T[b][el], acc[b]);
+                for (int b = 0; b < BT; b++)
+                    acc[b] = fma_rs(widen(Am), widen(T[b][el]), acc[b]);
             }
         }
     }
     // reduce over entry groups: R[g][b][m] reuses T
     __syncthreads();
-    V* Rb = &T[0][0];
+    W* Rb = kWide ? Tw : reinterpret_cast<W*>(&T[0][0]);
     if (live) {
 #pragma unroll
         for (int b = 0; b < BT; b++) Rb[(grp * BT + b) * nm + m] = acc[b];
@@ -125,9 +151,9 @@ lfc_integrate_kernel(const V* __restrict__ a, V* __restrict__ c,
     for (int u = t; u < BT * nm; u += blockDim.x) {
         const int b = u / nm;
         const int mm = u - b * nm;
-        V sum = vzero<V>();
+        W sum = vzero<W>();
         for (int g = 0; g < ngrp; g++) sum = vadd(sum, Rb[(g * BT + b) * nm + mm]);
-        if (b < nb) c[(size_t)(x0 + b) * Mmax + M0 + mm] = vscale(sum, dv);
+        if (b < nb) narrow(vscale(sum, (double)dv), c + (size_t)(x0 + b) * Mmax + M0 + mm);
     }
 }
 
@@ -200,20 +226,6 @@ lfc_add_kernel(const V* __restrict__ c, V* __restrict__ a,
             }
         }
     }
-}
-
-// FP64 value of a stored coefficient (single-precision data are widened)
-__device__ __forceinline__ double widen(double a) { return a; }
-__device__ __forceinline__ double widen(float a) { return (double)a; }
-template <typename S>
-__device__ __forceinline__ cplx<double> widen(cplx<S> a) {
-    return {(double)a.re, (double)a.im};
-}
-__device__ __forceinline__ void narrow(double a, double* o) { *o = a; }
-__device__ __forceinline__ void narrow(double a, float* o) { *o = (float)a; }
-template <typename S>
-__device__ __forceinline__ void narrow(cplx<double> a, cplx<S>* o) {
-    *o = {(S)a.re, (S)a.im};
 }
 
 // out[x, M0+i] = sum_j P[x, M0+j] dH[j][i]  (- eps[x] sum_j P dS[j][i]),
@@ -507,7 +519,9 @@ int gh_lfc_integrate(const gh_lfc* h, const void* a_dev, void* c_dev,
     // every (band, M) is written by the block of its atom, also for atoms
     // without grid points, so no clearing pass is needed
     hipStream_t st = (hipStream_t)stream;
-    const int bt = h->dtype == GH_C128 ? IntBT<cplx<double>>::value : IntBT<double>::value;
+    const int bt = h->dtype == GH_C128 ? IntBT<cplx<double>>::value
+                   : h->dtype == GH_C64 ? IntBT<cplx<float>>::value
+                                        : IntBT<double>::value;
     dim3 grid(h->natoms, (unsigned)((nbands + bt - 1) / bt));
     const void* ph = phase_table(h, q);
 #define ARGS(V, S) (const V*)a_dev, (V*)c_dev, h->entry_ptr, h->M_ptr, h->A_off, h->G_e, h->w_e, \
@@ -556,8 +570,14 @@ int gh_lfc_add(const gh_lfc* h, const void* c_dev, void* a_dev,
                 if (uniform8) hipLaunchKernelGGL((lfc_add_kernel<cplx<double>, double, 8>), grid, dim3(64), 0, st, ARGS(cplx<double>, double));
                 else hipLaunchKernelGGL((lfc_add_kernel<cplx<double>, double, 0>), grid, dim3(64), 0, st, ARGS(cplx<double>, double));
                 break;
-            case GH_F32: hipLaunchKernelGGL((lfc_add_kernel<float, float, 0>), grid, dim3(64), 0, st, ARGS(float, float)); break;
-            case GH_C64: hipLaunchKernelGGL((lfc_add_kernel<cplx<float>, float, 0>), grid, dim3(64), 0, st, ARGS(cplx<float>, float)); break;
+            case GH_F32:
+                if (uniform8) hipLaunchKernelGGL((lfc_add_kernel<float, float, 8>), grid, dim3(64), 0, st, ARGS(float, float));
+                else hipLaunchKernelGGL((lfc_add_kernel<float, float, 0>), grid, dim3(64), 0, st, ARGS(float, float));
+                break;
+            case GH_C64:
+                if (uniform8) hipLaunchKernelGGL((lfc_add_kernel<cplx<float>, float, 8>), grid, dim3(64), 0, st, ARGS(cplx<float>, float));
+                else hipLaunchKernelGGL((lfc_add_kernel<cplx<float>, float, 0>), grid, dim3(64), 0, st, ARGS(cplx<float>, float));
+                break;
         }
 #undef ARGS
         GH_CHECK_LAUNCH();

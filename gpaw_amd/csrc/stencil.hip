@@ -331,26 +331,21 @@ struct RingLayout {
 // SYM: the coefficients of +j and -j are equal along every axis (every
 // Laplacian); the paired real path then adds the two neighbours first and
 // keeps half the coefficients live.
+// One logical tile L of the ring stencil (band, x chunk, y tile): the
+// block's waves split into rg.ncomp compute and rg.nload loader wavefronts;
+// waves beyond those (a batched launch sized for a larger tile) only take
+// part in the barriers.
 template <typename V, typename S, int R, int PPT, bool HAS_V, bool HAS_SRC,
-          bool SYM = false>
-__global__ void __launch_bounds__(kRingMaxThreads)
-stencil_ring_kernel(const V* __restrict__ in, V* __restrict__ out,
-                    const S* __restrict__ vt, const V* __restrict__ src,
-                    S beta, CfdCoefs<S> cf, RingGeom rg, PhaseArgs<S> ph,
-                    RingLayout lay) {
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    V* ring = reinterpret_cast<V*>(smem_raw);
+          bool SYM>
+__device__ __forceinline__ void
+ring_tile(const V* __restrict__ in, V* __restrict__ out,
+          const S* __restrict__ vt, const V* __restrict__ src, S beta,
+          const CfdCoefs<S>& cf, const RingGeom& rg, const PhaseArgs<S>& ph,
+          int L, V* __restrict__ ring) {
     const CfdGeom& g = rg.g;
     const int D = rg.nload;
     const int P = R + D + 1;
     const int n2 = g.n2;
-    const int Lbase = xcd_swizzle(blockIdx.x, gridDim.x);
-    // persistent blocks take logical tiles Lbase, Lbase + gridDim.x, ...;
-    // every wave of the block sees the same sequence
-    for (int rd = 0; rd < rg.rounds; rd++) {
-    const int L = Lbase + rd * (int)gridDim.x;
-    if (L >= rg.nlogical) break;
-    if (rd > 0) ring_barrier();   // last reads of the previous tile done
     int band, xc, yt;
     ring_block_coords(rg, L, band, xc, yt);
     const S c0b = cf.eps ? cf.c0 - (S)cf.eps[band] : cf.c0;
@@ -366,6 +361,10 @@ stencil_ring_kernel(const V* __restrict__ in, V* __restrict__ out,
     const int lane = threadIdx.x & 63;
     const int xend = x1 + R;                 // planes [x0, xend) go through LDS
 
+    if (wave >= rg.ncomp + rg.nload) {
+        for (int x = x0; x < x1; x++) ring_barrier();
+        return;
+    }
     if (wave >= rg.ncomp) {
         // ---------------------------------------------------- loader --
         const int w = wave - rg.ncomp;
@@ -419,7 +418,7 @@ stencil_ring_kernel(const V* __restrict__ in, V* __restrict__ out,
             const int xp = x + R + D;
             if (xp < xend && ((xp - x0) % D) == w) issue(xp);
         }
-        continue;
+        return;
     }
 
     // ------------------------------------------------------ compute --
@@ -565,7 +564,7 @@ stencil_ring_kernel(const V* __restrict__ in, V* __restrict__ out,
         };
         if (g.pbc2) march(std::true_type{});
         else march(std::false_type{});
-        continue;
+        return;
     }
     if constexpr (is_complex<V>::value && PPT == 2) {
       {
@@ -705,7 +704,7 @@ stencil_ring_kernel(const V* __restrict__ in, V* __restrict__ out,
 #pragma unroll
             for (int j = 0; j < 2 * R; j++) q[j] = q[j + 1];
         }
-        continue;
+        return;
       }
     } else {
     // general path: complex data with Bloch phases (one point per thread)
@@ -816,7 +815,84 @@ stencil_ring_kernel(const V* __restrict__ in, V* __restrict__ out,
             for (int j = 0; j < 2 * R; j++) q[k][j] = q[k][j + 1];
     }
     }  // general path
-    }  // rounds
+}
+
+template <typename V, typename S, int R, int PPT, bool HAS_V, bool HAS_SRC,
+          bool SYM = false>
+__global__ void __launch_bounds__(kRingMaxThreads)
+stencil_ring_kernel(const V* __restrict__ in, V* __restrict__ out,
+                    const S* __restrict__ vt, const V* __restrict__ src,
+                    S beta, CfdCoefs<S> cf, RingGeom rg, PhaseArgs<S> ph,
+                    RingLayout lay) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    V* ring = reinterpret_cast<V*>(smem_raw);
+    const int Lbase = xcd_swizzle(blockIdx.x, gridDim.x);
+    // persistent blocks take logical tiles Lbase, Lbase + gridDim.x, ...;
+    // every wave of the block sees the same sequence
+    for (int rd = 0; rd < rg.rounds; rd++) {
+        const int L = Lbase + rd * (int)gridDim.x;
+        if (L >= rg.nlogical) break;
+        if (rd > 0) ring_barrier();   // last reads of the previous tile done
+        ring_tile<V, S, R, PPT, HAS_V, HAS_SRC, SYM>(in, out, vt, src, beta, cf, rg, ph, L,
+                                                      ring);
+    }
+}
+
+// Several independent grids (cells) in one persistent launch.  Cell c owns
+// the logical tiles [lbeg_c, lbeg_c + nlogical_c) of a global numbering;
+// its tile geometry (rows, prefetch depth, x chunks, wave split) comes from
+// the cell table, its arrays from the pointer table.  The block is sized
+// for the largest tile; LDS for the largest ring.
+constexpr int kMaxBatchCells = 8;
+
+template <typename S>
+struct RingCell {
+    CfdCoefs<S> cf;
+    RingGeom rg;
+    PhaseArgs<S> ph;
+    int lbeg;
+};
+
+template <typename V, typename S>
+struct BatchPtrs {
+    const V* in[kMaxBatchCells];
+    V* out[kMaxBatchCells];
+    const S* vt[kMaxBatchCells];
+};
+
+// Tiles are handed out in order (largest first) from a device counter, so
+// blocks that draw short tiles simply draw more of them.  The last block
+// to finish resets the counters for the next launch (no memset node, so
+// the launch can be captured in a graph); launches of one batch must not
+// overlap in time.
+template <typename V, typename S, int R, int PPT, bool HAS_V, bool SYM>
+__global__ void __launch_bounds__(kRingMaxThreads)
+stencil_ring_batched_kernel(const RingCell<S>* __restrict__ cells, BatchPtrs<V, S> ptr,
+                            int ncells, int ntotal, int* __restrict__ counters) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    __shared__ int s_tile;
+    V* ring = reinterpret_cast<V*>(smem_raw);
+    for (int rd = 0;; rd++) {
+        if (rd > 0) ring_barrier();   // the previous tile's LDS reads are done
+        if (threadIdx.x == 0) s_tile = atomicAdd(&counters[0], 1);
+        __syncthreads();
+        const int L = s_tile;
+        if (L >= ntotal) break;
+        int c = 0;
+        for (int k = 1; k < ncells; k++)
+            if (L >= cells[k].lbeg) c = k;
+        const RingCell<S>& cell = cells[c];
+        ring_tile<V, S, R, PPT, HAS_V, false, SYM>(ptr.in[c], ptr.out[c], ptr.vt[c], nullptr,
+                                                    (S)0, cell.cf, cell.rg, cell.ph,
+                                                    L - cell.lbeg, ring);
+    }
+    if (threadIdx.x == 0) {
+        __threadfence();
+        if (atomicAdd(&counters[1], 1) == (int)gridDim.x - 1) {
+            atomicExch(&counters[0], 0);
+            atomicExch(&counters[1], 0);
+        }
+    }
 }
 
 // ------------------------------------------------------ generic stencil --
@@ -1258,6 +1334,176 @@ int launch_generic(const gh_stencil* h, const void* in, void* out,
     return GH_OK;
 }
 
+
+// ------------------------------------------------------------ batches --
+// Launch constants of a batch of cells for the persistent ring launch.
+struct BatchPlan {
+    int dtype = 0;
+    int ncells = 0;
+    int R = 0;
+    bool has_v = false;
+    bool sym = true;
+    int order[kMaxBatchCells];      // table slot k holds cell order[k]
+    int threads = 0;
+    size_t lds = 0;
+    int nblocks = 0;
+    int rounds = 0;
+    int ntotal = 0;
+    void* table = nullptr;          // device RingCell<S>[ncells]
+    int* counters = nullptr;        // device: next tile, finished blocks
+    int rows[kMaxBatchCells];
+    int depth[kMaxBatchCells];
+};
+
+int device_cu_count(int* out) {
+    static int cached[64] = {0};
+    int dev = 0;
+    GH_CHECK_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) dev = 0;
+    if (!cached[dev])
+        GH_CHECK_HIP(hipDeviceGetAttribute(&cached[dev], hipDeviceAttributeMultiprocessorCount,
+                                           dev));
+    *out = cached[dev];
+    return GH_OK;
+}
+
+template <typename V, typename S, int R>
+int plan_batch(BatchPlan& bp, const gh_stencil* const* ops, const int64_t* nbands,
+               const int32_t* rows, const int32_t* depths) {
+    constexpr int PPT = 2;
+    const int vs = (int)sizeof(V);
+    std::vector<RingCell<S>> cells(bp.ncells);
+    std::vector<double> cost(bp.ncells);
+    std::vector<size_t> ldsv(bp.ncells);
+    int maxthr = 0;
+    size_t maxlds = 0;
+    for (int c = 0; c < bp.ncells; c++) {
+        const gh_stencil* h = ops[c];
+        const int n0 = (int)h->n[0], n1 = (int)h->n[1], n2 = (int)h->n[2];
+        GH_REQUIRE((n2 * vs) % kDmaBytes == 0 && n2 % 2 == 0, GH_ERR_UNSUPPORTED,
+                   "stencil_batch: cell %d rows of %d elements are not 16-byte multiples", c,
+                   n2);
+        auto lds_of = [&](int ty, int d) {
+            return (size_t)(R + d + 1) * (ty + 2 * R) * n2 * vs;
+        };
+        const int D = depths && depths[c] >= 2 ? depths[c] : 2;
+        int best = 0;
+        for (int ty = 1; ty <= n1; ty++) {
+            const int ncomp = ((ty * n2 + PPT - 1) / PPT + 63) / 64;
+            if (lds_of(ty, D) > kLdsBytes - 1024 || (ncomp + D) * 64 > kRingMaxThreads) break;
+            best = ty;
+        }
+        GH_REQUIRE(best > 0, GH_ERR_UNSUPPORTED, "stencil_batch: cell %d has no tile", c);
+        int ty = rows && rows[c] > 0 ? std::min<int>(rows[c], best) : best;
+        RingCell<S>& rc = cells[c];
+        CfdGeom& g = rc.rg.g;
+        g.n0 = n0; g.n1 = n1; g.n2 = n2;
+        g.pbc0 = h->pbc[0]; g.pbc1 = h->pbc[1]; g.pbc2 = h->pbc[2];
+        g.ty = ty;
+        g.nyt = (n1 + ty - 1) / ty;
+        g.lx = n0;
+        g.nxc = 1;
+        rc.rg.nload = D;
+        rc.rg.ncomp = ((ty * n2 + PPT - 1) / PPT + 63) / 64;
+        rc.rg.nbands = (int)nbands[c];
+        rc.rg.nlogical = (int)(nbands[c] * g.nyt);
+        rc.rg.rounds = 1;
+        rc.rg.nt_store = 1;
+        rc.rg.cplx_fast = 0;
+        rc.cf.c0 = (S)h->c0;
+        rc.cf.eps = nullptr;
+        for (int a = 0; a < 3; a++)
+            for (int j = 0; j < kMaxR; j++) {
+                rc.cf.cm[a][j] = (S)h->cm[a][j];
+                rc.cf.cp[a][j] = (S)h->cp[a][j];
+            }
+        rc.ph = phase_args<S>(nullptr);
+        for (int a = 0; a < 3; a++)
+            for (int j = 0; j < R; j++) bp.sym = bp.sym && h->cm[a][j] == h->cp[a][j];
+        ldsv[c] = lds_of(ty, D);
+        maxlds = std::max(maxlds, ldsv[c]);
+        maxthr = std::max(maxthr, (rc.rg.ncomp + D) * 64);
+        cost[c] = (double)n0 * ty * n2;
+        bp.rows[c] = ty;
+        bp.depth[c] = D;
+    }
+    // largest tiles first: the small ones fill the last rounds
+    for (int k = 0; k < bp.ncells; k++) bp.order[k] = k;
+    std::stable_sort(bp.order, bp.order + bp.ncells,
+                     [&](int a, int b) { return cost[a] > cost[b]; });
+    int ncu = 0;
+    int rc = device_cu_count(&ncu);
+    if (rc != GH_OK) return rc;
+    const int per_cu = std::max(1, std::min((int)(kLdsBytes / maxlds), 2048 / maxthr));
+    int nblocks = ncu * per_cu;
+    nblocks -= nblocks % kNumXCD;
+    int64_t ntotal = 0;
+    std::vector<RingCell<S>> sorted(bp.ncells);
+    for (int k = 0; k < bp.ncells; k++) {
+        RingCell<S> cl = cells[bp.order[k]];
+        cl.lbeg = (int)ntotal;
+        const int tiles = cl.rg.g.nyt * cl.rg.g.nxc;
+        cl.rg.bgroup = std::min(cl.rg.nbands, std::max(1, (nblocks / kNumXCD) / tiles));
+        ntotal += cl.rg.nlogical;
+        sorted[k] = cl;
+    }
+    GH_REQUIRE(ntotal < (1ll << 31), GH_ERR_SHAPE, "stencil_batch: too many tiles");
+    bp.ntotal = (int)ntotal;
+    bp.nblocks = (int)std::max<int64_t>(1, std::min<int64_t>(nblocks, ntotal));
+    bp.rounds = (int)((ntotal + bp.nblocks - 1) / bp.nblocks);   // on average
+    bp.threads = maxthr;
+    bp.lds = maxlds;
+    GH_CHECK_HIP(hipMalloc((void**)&bp.counters, 2 * sizeof(int)));
+    GH_CHECK_HIP(hipMemset(bp.counters, 0, 2 * sizeof(int)));
+    GH_CHECK_HIP(hipMalloc(&bp.table, sizeof(RingCell<S>) * bp.ncells));
+    GH_CHECK_HIP(hipMemcpy(bp.table, sorted.data(), sizeof(RingCell<S>) * bp.ncells,
+                           hipMemcpyHostToDevice));
+    // the dynamic LDS limit is a property of the kernel: set once here
+#define GH_BATCH_ATTR(HV, SY)                                                   \
+    GH_CHECK_HIP(hipFuncSetAttribute(                                           \
+        (const void*)stencil_ring_batched_kernel<V, S, R, PPT, HV, SY>,         \
+        hipFuncAttributeMaxDynamicSharedMemorySize, (int)(kLdsBytes - 1024)))
+    GH_BATCH_ATTR(true, true);
+    GH_BATCH_ATTR(true, false);
+    GH_BATCH_ATTR(false, true);
+    GH_BATCH_ATTR(false, false);
+#undef GH_BATCH_ATTR
+    return GH_OK;
+}
+
+template <typename V, typename S, int R>
+int launch_batch(const BatchPlan& bp, const void* const* in, void* const* out,
+                 const void* const* vt, hipStream_t st) {
+    constexpr int PPT = 2;
+    BatchPtrs<V, S> p;
+    for (int k = 0; k < kMaxBatchCells; k++) {
+        p.in[k] = nullptr;
+        p.out[k] = nullptr;
+        p.vt[k] = nullptr;
+    }
+    for (int k = 0; k < bp.ncells; k++) {
+        const int c = bp.order[k];
+        p.in[k] = (const V*)in[c];
+        p.out[k] = (V*)out[c];
+        p.vt[k] = vt ? (const S*)vt[c] : nullptr;
+    }
+    const RingCell<S>* tab = (const RingCell<S>*)bp.table;
+    dim3 grid((unsigned)bp.nblocks), block((unsigned)bp.threads);
+#define GH_BATCH_LAUNCH(HV, SY)                                                 \
+    hipLaunchKernelGGL((stencil_ring_batched_kernel<V, S, R, PPT, HV, SY>), grid, block, \
+                       bp.lds, st, tab, p, bp.ncells, bp.ntotal, bp.counters)
+    if (bp.has_v) {
+        if (bp.sym) GH_BATCH_LAUNCH(true, true);
+        else GH_BATCH_LAUNCH(true, false);
+    } else {
+        if (bp.sym) GH_BATCH_LAUNCH(false, true);
+        else GH_BATCH_LAUNCH(false, false);
+    }
+#undef GH_BATCH_LAUNCH
+    GH_CHECK_LAUNCH();
+    return GH_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1398,6 +1644,106 @@ int gh_stencil_relax(const gh_stencil* h, int relax_method, void* f_dev,
         GH_CHECK_HIP(hipMemcpyAsync(f_dev, cur, bytes, hipMemcpyDeviceToDevice, st));
     }
     return GH_OK;
+}
+
+
+struct gh_stencil_batch {
+    BatchPlan plan;
+};
+
+int gh_stencil_batch_create(gh_stencil_batch** out, int ncells, const gh_stencil* const* ops,
+                            const int64_t* nbands, const int32_t* rows,
+                            const int32_t* depths, int with_potential) {
+    GH_REQUIRE(out && ops && nbands, GH_ERR_ARG, "stencil_batch_create: null argument");
+    GH_REQUIRE(ncells >= 1 && ncells <= kMaxBatchCells, GH_ERR_ARG,
+               "stencil_batch_create: %d cells (1..%d)", ncells, kMaxBatchCells);
+    const int dt = ops[0] ? ops[0]->dtype : -1;
+    GH_REQUIRE(dt == GH_F32 || dt == GH_F64, GH_ERR_UNSUPPORTED,
+               "stencil_batch_create: real float32 / float64 cells only");
+    const int R = ops[0]->R;
+    for (int c = 0; c < ncells; c++) {
+        GH_REQUIRE(ops[c] && ops[c]->dtype == dt && ops[c]->cfd && ops[c]->R == R,
+                   GH_ERR_ARG,
+                   "stencil_batch_create: cell %d: every cell needs an axis-aligned stencil "
+                   "of the same dtype and radius", c);
+        GH_REQUIRE(nbands[c] >= 1 && nbands[c] < (1 << 24), GH_ERR_ARG,
+                   "stencil_batch_create: cell %d: bad band count", c);
+    }
+    GH_REQUIRE(R >= 1 && R <= 3, GH_ERR_UNSUPPORTED, "stencil_batch_create: radius %d", R);
+    auto* b = new gh_stencil_batch();
+    BatchPlan& bp = b->plan;
+    bp.dtype = dt;
+    bp.ncells = ncells;
+    bp.R = R;
+    bp.has_v = with_potential != 0;
+    int rc = GH_ERR_UNSUPPORTED;
+#define GH_PLAN(V, S)                                                           \
+    switch (R) {                                                               \
+        case 1: rc = plan_batch<V, S, 1>(bp, ops, nbands, rows, depths); break; \
+        case 2: rc = plan_batch<V, S, 2>(bp, ops, nbands, rows, depths); break; \
+        case 3: rc = plan_batch<V, S, 3>(bp, ops, nbands, rows, depths); break; \
+    }
+    if (dt == GH_F32) { GH_PLAN(float, float) } else { GH_PLAN(double, double) }
+#undef GH_PLAN
+    if (rc != GH_OK) {
+        if (bp.table) (void)hipFree(bp.table);
+        if (bp.counters) (void)hipFree(bp.counters);
+        delete b;
+        return rc;
+    }
+    *out = b;
+    return GH_OK;
+}
+
+int gh_stencil_batch_destroy(gh_stencil_batch* b) {
+    if (!b) return GH_OK;
+    if (b->plan.table) (void)hipFree(b->plan.table);
+    if (b->plan.counters) (void)hipFree(b->plan.counters);
+    delete b;
+    return GH_OK;
+}
+
+int gh_stencil_batch_info(const gh_stencil_batch* b, int64_t* info, int32_t* rows,
+                          int32_t* depths) {
+    GH_REQUIRE(b && info, GH_ERR_ARG, "stencil_batch_info: null argument");
+    const BatchPlan& bp = b->plan;
+    info[0] = bp.nblocks;
+    info[1] = bp.threads;
+    info[2] = (int64_t)bp.lds;
+    info[3] = bp.ntotal;
+    info[4] = bp.rounds;
+    for (int c = 0; c < bp.ncells; c++) {
+        if (rows) rows[c] = bp.rows[c];
+        if (depths) depths[c] = bp.depth[c];
+    }
+    return GH_OK;
+}
+
+int gh_stencil_batch_apply(const gh_stencil_batch* b, const void* const* in_dev,
+                           void* const* out_dev, const void* const* vt_dev, void* stream) {
+    GH_REQUIRE(b && in_dev && out_dev, GH_ERR_ARG, "stencil_batch_apply: null argument");
+    const BatchPlan& bp = b->plan;
+    GH_REQUIRE(!bp.has_v || vt_dev, GH_ERR_ARG,
+               "stencil_batch_apply: the batch was made with potentials");
+    for (int c = 0; c < bp.ncells; c++) {
+        GH_REQUIRE(in_dev[c] && out_dev[c] && in_dev[c] != out_dev[c], GH_ERR_ARG,
+                   "stencil_batch_apply: cell %d: null or aliased arrays", c);
+        auto al = [](const void* p) { return ((uintptr_t)p & 15u) == 0; };
+        GH_REQUIRE(al(in_dev[c]) && al(out_dev[c]) && (!bp.has_v || (vt_dev[c] && al(vt_dev[c]))),
+                   GH_ERR_ARG, "stencil_batch_apply: cell %d: arrays must be 16-byte aligned",
+                   c);
+    }
+    hipStream_t st = (hipStream_t)stream;
+    const void* const* vt = bp.has_v ? vt_dev : nullptr;
+#define GH_RUN(V, S)                                                            \
+    switch (bp.R) {                                                            \
+        case 1: return launch_batch<V, S, 1>(bp, in_dev, out_dev, vt, st);      \
+        case 2: return launch_batch<V, S, 2>(bp, in_dev, out_dev, vt, st);      \
+        case 3: return launch_batch<V, S, 3>(bp, in_dev, out_dev, vt, st);      \
+    }
+    if (bp.dtype == GH_F32) { GH_RUN(float, float) } else { GH_RUN(double, double) }
+#undef GH_RUN
+    return GH_ERR_UNSUPPORTED;
 }
 
 }  // extern "C"

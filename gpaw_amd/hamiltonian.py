@@ -19,9 +19,21 @@ class FDHamiltonian:
         self.gd = gd
         self.kin = kin
         self.dtype = dtype
-        self.vt_G = torch.as_tensor(np.asarray(vt_G) if not isinstance(
-            vt_G, torch.Tensor) else vt_G, device='cuda').to(
-                real_dtype(dtype)).contiguous()
+        vt64 = torch.as_tensor(np.asarray(vt_G) if not isinstance(
+            vt_G, torch.Tensor) else vt_G, device='cuda').to(torch.float64)
+        rdt = real_dtype(dtype)
+        self.stencil_offset = 0.0
+        if rdt == torch.float32:
+            # The single-precision stencil's rounded coefficients do not sum
+            # to zero: T_32 = T + delta on constants, a uniform shift of every
+            # eigenvalue by delta (~1e-6 Ha for h = 0.2 A).  delta is known
+            # exactly (the FP64 sum of the FP32 coefficients), and the centre
+            # coefficient multiplies psi(g) as the potential does, so it is
+            # removed through the potential before that is rounded.
+            coef32 = np.asarray(kin.coef_p, np.float64).astype(np.float32)
+            self.stencil_offset = float(coef32.astype(np.float64).sum())
+            vt64 = vt64 - self.stencil_offset
+        self.vt_G = vt64.to(rdt).contiguous()
         self.pt = pt
         self._lib = _lib.lib()
         self.dS_flat = None

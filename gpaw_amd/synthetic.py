@@ -219,3 +219,21 @@ class System:
         self.H = FDHamiltonian(gd, self.kin, vt, self.pt, w.dH, dtype=dtype)
         self.pc = Preconditioner(gd, self.kin, dtype)
         self.phase_cd = None if k_c is None else gd.phase_cd(k_c)
+
+
+# Six independent cells of uneven size: grid points per axis, bands, atoms
+# (h = 0.2 A each; potentials and projectors as in full_hpsi_geometry).
+BATCH_CELLS = ((40, 16, 8), (56, 48, 16), (72, 96, 32), (96, 192, 48),
+               (120, 384, 80), (128, 512, 96))
+
+
+def batch_cells(cells=BATCH_CELLS, dtype=np.float32):
+    """Geometry, grid descriptor and single-precision System per cell."""
+    from .grid_descriptor import GridDescriptor
+    out = []
+    for N, nb, natoms in cells:
+        w = full_hpsi_geometry(N, natoms)
+        gd = GridDescriptor((N,) * 3, (w.L,) * 3)
+        vt = potential(gd, w.pos_av)
+        out.append((w, gd, vt, nb, System(w, gd, vt, dtype)))
+    return out

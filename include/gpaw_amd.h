@@ -41,6 +41,7 @@ extern "C" {
 typedef struct gh_stencil gh_stencil;
 typedef struct gh_transformer gh_transformer;
 typedef struct gh_lfc gh_lfc;
+typedef struct gh_stencil_batch gh_stencil_batch;
 
 /* Last error message of the calling thread ("" if none). */
 const char* gh_last_error(void);
@@ -107,6 +108,37 @@ int gh_stencil_set_tile(gh_stencil* h, int rows, int planes, int depth);
 int gh_stencil_relax(const gh_stencil* h, int relax_method, void* f_dev,
                      const void* s_dev, void* work_dev, int nrelax, double w,
                      void* stream);
+
+/* ------------------------------------------------------ batched cells --
+ * Several independent grids ("cells") of different sizes and band counts in
+ * one persistent launch of the LDS-ring stencil: out_c = L_c in_c
+ * (+ vt_c .* in_c).  No single reference counterpart: each cell is one
+ * Operator.apply (c/operators.c:250-321; the reference's device-pointer
+ * form c/gpu/operators.c:350-391 launches one grid per call); the batch
+ * issues all of them as one kernel whose blocks walk a global tile list,
+ * largest tiles first.  Cells: 1..8 real float32 or float64 axis-aligned
+ * stencils (radius <= 3) with rows of a multiple of 16 bytes.  rows[c],
+ * depths[c]: tile rows and prefetch depth per cell (0 / NULL: the largest
+ * tile at depth 2), e.g. from an autotuner.  with_potential: apply takes
+ * one potential per cell.  The plan (tile table and a tile counter) lives
+ * on the device; the apply call only passes pointers, so it can be captured
+ * in a HIP graph.  Blocks draw tiles from the counter (largest tiles first)
+ * and the last block resets it: launches of one batch must not overlap in
+ * time (one stream, or ordered streams). */
+int gh_stencil_batch_create(gh_stencil_batch** out, int ncells,
+                            const gh_stencil* const* ops, const int64_t* nbands,
+                            const int32_t* rows, const int32_t* depths,
+                            int with_potential);
+int gh_stencil_batch_destroy(gh_stencil_batch* b);
+/* info[5] = {blocks, threads per block, LDS bytes, tiles, tiles per block};
+ * rows / depths (may be NULL) receive the tile chosen per cell. */
+int gh_stencil_batch_info(const gh_stencil_batch* b, int64_t* info,
+                          int32_t* rows, int32_t* depths);
+/* in_dev[c], out_dev[c]: (nbands_c, n0, n1, n2) of cell c, 16-byte aligned;
+ * vt_dev[c]: (n0, n1, n2) potential of cell c (with_potential). */
+int gh_stencil_batch_apply(const gh_stencil_batch* b, const void* const* in_dev,
+                           void* const* out_dev, const void* const* vt_dev,
+                           void* stream);
 
 /* -------------------------------------------------------- transformers --
  * Replaces cgpaw.Transformer (reference c/transformers.c:202-252, created
