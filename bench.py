@@ -59,6 +59,8 @@ def parse():
                    help='skip the pseudo-density / Poisson leg')
     p.add_argument('--no-transformers', action='store_true',
                    help='skip the restrict / interpolate leg')
+    p.add_argument('--no-learned', action='store_true',
+                   help='skip the learned-vs-multigrid preconditioner leg')
     p.add_argument('--no-batched', action='store_true',
                    help='skip the batched uneven cells leg')
     p.add_argument('--no-complex64', action='store_true',
@@ -91,48 +93,108 @@ def build_workload(N, natoms, nbands):
     return w, gd, pt, vt, kin, H, psi, out
 
 
+def _cpu_model():
+    try:
+        with open('/proc/cpuinfo') as fh:
+            for line in fh:
+                if line.startswith('model name'):
+                    return line.split(':', 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def _set_omp_threads(n):
+    """Thread count of the OpenMP runtime the reference C build uses."""
+    import ctypes
+    try:
+        ctypes.CDLL('libgomp.so.1').omp_set_num_threads(int(n))
+        return True
+    except OSError:
+        return False
+
+
+def _median_time(fn, min_runs=5, budget=8.0):
+    fn()   # warm-up
+    times = []
+    t_end = time.perf_counter() + budget
+    while len(times) < min_runs or (time.perf_counter() < t_end and
+                                    len(times) < 9):
+        t0 = time.perf_counter()
+        fn()
+        times.append(time.perf_counter() - t0)
+    return float(np.median(times)), len(times)
+
+
 def cpu_baseline(w, gd_args, vt, nb_sample):
-    """Reference C path (oracle/_ref) on `nb_sample` bands: Operator.apply,
-    the NumPy potential multiply, LFC.integrate, P dH, LFC.add."""
+    """The reference's own C kernels (oracle/_ref, built from its sources)
+    on the host cores, at one thread (the reference's default,
+    gpaw/__init__.py:53-54) and at every core this process may use: full
+    H psi (Operator.apply, the NumPy potential multiply, LFC.integrate,
+    P dH, LFC.add) on a sample of the bands, and per band the V-cycle
+    (fd_precond) and the nn = 3 restriction / interpolation 112^3 <-> 56^3.
+    Median of >= 5 runs after a warm-up."""
     from oracle import refbuild, grid as og, spline as osp
     from oracle.stencil import laplace_stencil
     if not refbuild.available():
         return None
     N, L = gd_args
     ogd = og.Grid((N,) * 3, (L,) * 3)
+    ogd1 = ogd.coarsen()
     c, o = laplace_stencil(ogd.h_cv, -0.5, 3)
     op = refbuild.Operator(c, o, ogd, True)
     spl = [[osp.Spline(l, rc, f) for l, rc, f in p] for p in w.proj]
     lfc = refbuild.LFC(ogd, spl, w.spos_ac)
+    rst = refbuild.Transformer(ogd, ogd1, 3, True)
+    itp = refbuild.Transformer(ogd1, ogd, 3, True)
     rng = np.random.default_rng(1)
-    psi = rng.standard_normal((nb_sample, N, N, N))
-    out = np.empty_like(psi)
+    try:
+        ncores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        ncores = os.cpu_count()
+    legs = {}
+    hpsi = {}
+    for threads in (1, ncores):
+        if not _set_omp_threads(threads):
+            return None
+        nbs = max(2, nb_sample // 4) if threads == 1 else nb_sample
+        psi = rng.standard_normal((nbs, N, N, N))
+        out = np.empty_like(psi)
 
-    def step():
-        op.apply(psi, out, None)
-        out.__iadd__(psi * vt)
-        P = lfc.integrate(psi)
-        Q = np.empty_like(P)
-        for a in range(w.natoms):
-            Q[:, 8 * a:8 * a + 8] = P[:, 8 * a:8 * a + 8] @ w.dH[a]
-        lfc.add(out, Q)
-    step()  # warm-up
-    times = []
-    t_end = time.perf_counter() + 20.0
-    while len(times) < 5 or (time.perf_counter() < t_end and len(times) < 9):
-        t0 = time.perf_counter()
-        step()
-        times.append(time.perf_counter() - t0)
-        if time.perf_counter() > t_end + 10:
-            break
-    t = float(np.median(times))
-    threads = int(os.environ.get('OMP_NUM_THREADS', os.cpu_count()))
-    return {'value': nb_sample / t, 'unit': 'band-Hψ/s',
-            'cores': threads, 'kind': 'reference',
-            'sample': ('%d of the 256 bands, same 112^3 grid, 64 atoms; '
-                       'reference c/operators.c + c/lfc.c (OpenMP over '
-                       'bands) + NumPy potential/dH; median of %d runs '
-                       '(%.2f s each)' % (nb_sample, len(times), t))}
+        def step():
+            op.apply(psi, out, None)
+            out.__iadd__(psi * vt)
+            P = lfc.integrate(psi)
+            Q = np.empty_like(P)
+            for a in range(w.natoms):
+                Q[:, 8 * a:8 * a + 8] = P[:, 8 * a:8 * a + 8] @ w.dH[a]
+            lfc.add(out, Q)
+        t, nrun = _median_time(step)
+        hpsi[threads] = (nbs / t, nbs, nrun, t)
+        nv = 4
+        R = rng.standard_normal((nv, N, N, N))
+        tv, _ = _median_time(lambda: refbuild.fd_precond(ogd, c, o, R))
+        coarse = np.empty((nv,) + tuple(ogd1.n_c))
+        tr, _ = _median_time(lambda: rst.apply(R, coarse, None))
+        ti, _ = _median_time(lambda: itp.apply(coarse, R, None))
+        legs['%d_threads' % threads] = {
+            'vcycle_ms_per_band': round(1e3 * tv / nv, 3),
+            'restrict_nn3_ms_per_band': round(1e3 * tr / nv, 3),
+            'interpolate_nn3_ms_per_band': round(1e3 * ti / nv, 3)}
+        del psi, out
+    v_all, nb_all, nrun, t_all = hpsi[ncores]
+    v_one, nb_one, nrun1, t_one = hpsi[1]
+    return {'value': round(v_all, 2), 'unit': 'band-Hψ/s',
+            'cores': ncores, 'kind': 'reference',
+            'cpu_model': _cpu_model(),
+            'single_thread_value': round(v_one, 2),
+            'sample': ('full H psi on %d of the 256 bands (%d at 1 thread), '
+                       'same 112^3 grid, 64 atoms; reference c/operators.c '
+                       '+ c/lfc.c (OpenMP over bands) + NumPy potential / '
+                       'dH; median of %d runs (%.2f s each at %d threads, '
+                       '%.2f s at 1 thread)' % (nb_all, nb_one, nrun, t_all,
+                                                ncores, t_one)),
+            'legs': legs}
 
 
 def fp64_ritz_values(sys64, psit, dv):
@@ -161,7 +223,8 @@ def fp64_ritz_values(sys64, psit, dv):
 
 def eigensolver_run(w, gd, vt, nb, nconv, psi0, sweeps, dtype=np.float64,
                     k_c=None, solver='rmm-diis', tol=1e-8, polish=0,
-                    polish_tol=1e-13, final_fp64=None):
+                    polish_tol=1e-13, final_fp64=None, make_pc=None,
+                    sysm=None):
     """RMM-DIIS (niter=3, all bands in one block) or Davidson (niter=2) with
     the multigrid preconditioner, wave functions and grid work in `dtype`
     (float64 / complex128: the plain FP64 form; float32 / complex64: the
@@ -177,9 +240,11 @@ def eigensolver_run(w, gd, vt, nb, nconv, psi0, sweeps, dtype=np.float64,
     from gpaw_amd import synthetic as syn
     from gpaw_amd._arrays import torch_dtype
     from gpaw_amd.eigensolvers import Davidson, RMMDIIS, FDWaveFunctions
-    sysm = syn.System(w, gd, vt, dtype, k_c)
+    if sysm is None:
+        sysm = syn.System(w, gd, vt, dtype, k_c)
+    pc = sysm.pc if make_pc is None else make_pc(sysm)
     wfs = FDWaveFunctions(gd, psi0.to(torch_dtype(dtype)).contiguous(),
-                          sysm.pt, sysm.pc, sysm.phase_cd)
+                          sysm.pt, pc, sysm.phase_cd)
     es = RMMDIIS() if solver == 'rmm-diis' else Davidson()
     weights = torch.zeros(nb, dtype=torch.float64, device='cuda')
     weights[:nconv] = 1.0
@@ -215,7 +280,8 @@ def eigensolver_run(w, gd, vt, nb, nconv, psi0, sweeps, dtype=np.float64,
     res = {'solver': 'RMMDIIS(niter=3, all bands in one block)'
            if solver == 'rmm-diis' else repr(es),
            'dtype': str(np.dtype(dtype)),
-           'precond': 'multigrid V-cycle (%s)' % np.dtype(dtype),
+           'precond': ('multigrid V-cycle (%s)' % np.dtype(dtype)
+                       if make_pc is None else type(pc).__name__),
            'bands': nb, 'bands_converged': nconv,
            'sweeps': len(errs), 'seconds': round(t, 3),
            'ms_per_sweep': round(1e3 * t / max(len(errs), 1), 2),
@@ -227,8 +293,11 @@ def eigensolver_run(w, gd, vt, nb, nconv, psi0, sweeps, dtype=np.float64,
     if eps_rr is not None:
         res['fp64_rayleigh_ritz_ms'] = round(1e3 * t_rr, 2)
         res['seconds_to_1e-8_with_fp64_ritz'] = round(t_tol + t_rr, 3)
+    if hasattr(pc, 'fallback_active'):
+        res['learned_calls'] = pc.calls
+        res['fallback_to_vcycle'] = bool(pc.fallback_active)
     eps = (eps_tol, wfs.eps[:nconv].cpu().numpy(), eps_rr)
-    del wfs, sysm, es
+    del wfs, sysm, es, pc
     torch.cuda.empty_cache()
     return res, eps
 
@@ -292,6 +361,56 @@ def eigensolver_leg(w, gd, vt, H, nb, nconv, sweeps, k_c=None,
         rd['eigenvalue_max_abs_err_at_1e-8_Ha'] = float(abs(ed[0] - ref).max())
         out['davidson_fp64'] = rd
     del psi0
+    torch.cuda.empty_cache()
+    return out
+
+
+def learned_leg(weights, N=144, natoms=128, nb=512, nconv=128, sweeps=30):
+    """The second metric at 144^3 with 512 bands and 128 atoms (generated
+    as the 112^3 workload): FP64 RMM-DIIS to err <= 1e-8 from one LCAO-like
+    start with the multigrid V-cycle and with the learned preconditioner
+    (U-Net on the matrix cores, the V-cycle as fallback judged by the
+    residual norm).  Error over the lowest `nconv` bands."""
+    import torch
+    from gpaw_amd import synthetic as syn
+    from gpaw_amd.grid_descriptor import GridDescriptor
+    from gpaw_amd.learned import LearnedPreconditioner
+    w = syn.full_hpsi_geometry(N, natoms)
+    gd = GridDescriptor((N,) * 3, (w.L,) * 3)
+    vt = syn.potential(gd, w.pos_av)
+    sysm = syn.System(w, gd, vt, np.float64)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    psi0 = syn.lcao_start(gd, w, sysm.H, nb)
+    torch.cuda.synchronize()
+    out = {'workload': '%d bands, %d^3 grid, %d atoms, FP64' % (nb, N,
+                                                                 natoms),
+           'start': 'LCAO-like, %.2f s, not in the timed region'
+                    % (time.perf_counter() - t0),
+           'weights': os.path.relpath(weights, ROOT)}
+    torch.cuda.empty_cache()
+    rmg, emg = eigensolver_run(w, gd, vt, nb, nconv, psi0, sweeps,
+                               sysm=sysm)
+    torch.cuda.empty_cache()
+
+    def make(s):
+        return LearnedPreconditioner(gd, s.kin, vt, weights, fallback=s.pc)
+    rnn, enn = eigensolver_run(w, gd, vt, nb, nconv, psi0, sweeps,
+                               sysm=sysm, make_pc=make)
+    rnn['eigenvalue_max_abs_diff_vs_multigrid_Ha'] = float(
+        abs(enn[0] - emg[0]).max())
+    out['multigrid'] = rmg
+    out['learned'] = rnn
+    if rmg['seconds_to_1e-8'] and rnn['seconds_to_1e-8']:
+        out['speedup_learned_vs_multigrid'] = round(
+            rmg['seconds_to_1e-8'] / rnn['seconds_to_1e-8'], 2)
+    # cost of one preconditioner application on all bands
+    lp = make(sysm)
+    R = torch.randn((nb,) + tuple(gd.n_c), dtype=torch.float64,
+                    device='cuda')
+    out['vcycle_ms_all_bands'] = round(_time_ms(lambda: sysm.pc(R), 3), 2)
+    out['unet_ms_all_bands'] = round(_time_ms(lambda: lp(R), 3), 2)
+    del psi0, R, lp, sysm
     torch.cuda.empty_cache()
     return out
 
@@ -613,6 +732,12 @@ def main():
                 args.rmm_sweeps, k_c=(0.25, 0.0, 0.0))
         rmm['dense'] = dense_leg(nb, ngpts)
 
+    learned = None
+    weights = os.path.join(ROOT, 'gpaw_amd', 'data', 'unet_precond.npz')
+    if not args.no_learned and os.path.exists(weights):
+        out = None
+        learned = learned_leg(weights)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(w, (N, w.L), vt, args.cpu_bands)
@@ -648,6 +773,7 @@ def main():
         'single_precision_complex_k': c3,
         'transformers': trans,
         'batched_cells': batched,
+        'learned_vs_multigrid': learned,
         'density_and_poisson': scf_next,
         'cpu_baseline': cpu,
     }

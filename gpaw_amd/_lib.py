@@ -70,6 +70,15 @@ _SIGNATURES = {
     'gh_gram_work_bytes': (I64, [_c.c_int, I64, I64, I64, _c.c_int]),
     'gh_gram': (_c.c_int, [_c.c_int, I64, I64, I64, P, I64, P, I64, D,
                            _c.c_int, _c.c_int, P, P, I64, P]),
+    'gh_unet_conv3_weight_elems': (I64, [_c.c_int, _c.c_int]),
+    'gh_unet_pack_conv3': (_c.c_int, [_c.c_int, _c.c_int, P, P]),
+    'gh_unet_conv3': (_c.c_int, [_c.c_int, _c.c_int, _c.c_int, _c.c_int,
+                                 _c.c_int, PI64, P, P, P, P, P, D, P, P,
+                                 _c.c_int, P, P]),
+    'gh_unet_pool2': (_c.c_int, [_c.c_int, _c.c_int, PI64, P, P, P]),
+    'gh_unet_map1': (_c.c_int, [_c.c_int, _c.c_int, I64, P, P, P, P, P]),
+    'gh_unet_head': (_c.c_int, [_c.c_int, _c.c_int, _c.c_int, I64, P, P, D,
+                                P, P, P]),
     'gh_spline_coefs': (_c.c_int, [_c.c_int, D, PD, PD]),
     'gh_spline_to_grid': (I64, [_c.c_int, _c.c_int, D, PD, PI64, PI64, PD,
                                 PD, PI64, PI64, PD, PI32]),

@@ -26,6 +26,7 @@
 #include <string>
 #include <vector>
 #include <algorithm>
+#include <mutex>
 
 #include "common.h"
 #include "internal.h"
@@ -1095,6 +1096,33 @@ int launch_cfd_r(const gh_stencil* h, const void* in, void* out,
 // fall back to the register kernel.
 constexpr size_t kLdsBytes = 160 * 1024;
 
+// Compute units of the current device (queried once per device).
+int device_cu_count(int* out) {
+    static int cached[64] = {0};
+    int dev = 0;
+    GH_CHECK_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) dev = 0;
+    if (!cached[dev])
+        GH_CHECK_HIP(hipDeviceGetAttribute(&cached[dev], hipDeviceAttributeMultiprocessorCount,
+                                           dev));
+    *out = cached[dev];
+    return GH_OK;
+}
+
+// Raise a kernel's dynamic LDS limit to the whole 160 KB once per process
+// (not per launch).
+std::mutex g_attr_mutex;
+std::vector<const void*> g_attr_done;
+
+int ensure_max_lds(const void* fn, int bytes) {
+    std::lock_guard<std::mutex> lock(g_attr_mutex);
+    if (std::find(g_attr_done.begin(), g_attr_done.end(), fn) != g_attr_done.end())
+        return GH_OK;
+    GH_CHECK_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+    g_attr_done.push_back(fn);
+    return GH_OK;
+}
+
 template <typename V, typename S, int R, int PPT>
 int launch_ring_r(const gh_stencil* h, const void* in, void* out,
                   int64_t nbands, const double* phase_cd, const void* vt,
@@ -1158,9 +1186,9 @@ int launch_ring_r(const gh_stencil* h, const void* in, void* out,
         // one block per CU: when the bands x y-tiles do not cover the CUs,
         // split the x-march into chunks (each chunk re-reads 2R planes),
         // keeping chunks at least 4R planes long
-        int ncu = 0, dev = 0;
-        GH_CHECK_HIP(hipGetDevice(&dev));
-        GH_CHECK_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+        int ncu = 0;
+        int rc = device_cu_count(&ncu);
+        if (rc != GH_OK) return rc;
         const int64_t base = nbands * (int64_t)g.nyt;
         int nxc = 1;
         while (base * nxc < ncu && (n0 + nxc) / (nxc + 1) >= 4 * R) nxc++;
@@ -1182,9 +1210,9 @@ int launch_ring_r(const gh_stencil* h, const void* in, void* out,
         // group, so all blocks of an XCD stay on the same planes
         int64_t n = h->persist;
         if (n < kNumXCD) {
-            int ncu = 0, dev = 0;
-            GH_CHECK_HIP(hipGetDevice(&dev));
-            GH_CHECK_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+            int ncu = 0;
+            int rc = device_cu_count(&ncu);
+            if (rc != GH_OK) return rc;
             const int per_cu = std::max<int>(1, (int)(kLdsBytes / std::max<size_t>(lds, 1)));
             n = (int64_t)ncu * per_cu;
         }
@@ -1229,8 +1257,8 @@ int launch_ring_r(const gh_stencil* h, const void* in, void* out,
 #define GH_LAUNCH_RING_S(HV, HS, SY)                                           \
     do {                                                                       \
         auto kfn = stencil_ring_kernel<V, S, R, PPT, HV, HS, SY>;              \
-        GH_CHECK_HIP(hipFuncSetAttribute((const void*)kfn,                     \
-            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));            \
+        int arc = ensure_max_lds((const void*)kfn, (int)kLdsBytes);            \
+        if (arc != GH_OK) return arc;                                          \
         hipLaunchKernelGGL(kfn, grid, block, lds, st, (const V*)in, (V*)out,   \
                            vtp, srcp, (S)beta, cf, rg, ph, lay);               \
     } while (0)
@@ -1355,17 +1383,6 @@ struct BatchPlan {
     int depth[kMaxBatchCells];
 };
 
-int device_cu_count(int* out) {
-    static int cached[64] = {0};
-    int dev = 0;
-    GH_CHECK_HIP(hipGetDevice(&dev));
-    if (dev < 0 || dev >= 64) dev = 0;
-    if (!cached[dev])
-        GH_CHECK_HIP(hipDeviceGetAttribute(&cached[dev], hipDeviceAttributeMultiprocessorCount,
-                                           dev));
-    *out = cached[dev];
-    return GH_OK;
-}
 
 template <typename V, typename S, int R>
 int plan_batch(BatchPlan& bp, const gh_stencil* const* ops, const int64_t* nbands,

@@ -170,6 +170,25 @@ class RMMDIIS(Eigensolver):
         G = blas.multi_gram([dR, Rb], scale=wfs.dv)
         if G.is_complex():
             G = G.real
+        judge = getattr(wfs.preconditioner, 'judge', None)
+        if judge is not None:
+            # fraction of |R|^2 left after the line search along dpsit: a
+            # preconditioner with a fallback may reject its step per band;
+            # those bands are redone with the fallback
+            rho = 1.0 - G[:, 0, 1] ** 2 / (G[:, 0, 0] * G[:, 1, 1])
+            bad = judge(rho)
+            if bad is not None and bool(bad.any()):
+                idx = bad.nonzero()[:, 0]
+                Rbad = Rb[idx].contiguous()
+                d_fb = wfs.preconditioner.fallback(Rbad, kpt)
+                dR_fb = torch.empty_like(d_fb)
+                self.residual_of(ham, wfs, d_fb, eps[idx].contiguous(),
+                                 dR_fb)
+                dpsit[idx] = d_fb
+                dR[idx] = dR_fb
+                G = blas.multi_gram([dR, Rb], scale=wfs.dv)
+                if G.is_complex():
+                    G = G.real
         error = G[:, 1, 1]
         lam = -G[:, 0, 1] / G[:, 0, 0]
         if self.limit_lambda:

@@ -327,6 +327,42 @@ int gh_gram(int dtype, int64_t m, int64_t n, int64_t k, const void* A_dev,
             int hermitian, int conj_out, double* C_dev, void* work_dev,
             int64_t work_bytes, void* stream);
 
+/* ---------------------------------------------- learned preconditioner --
+ * Layers of the 3D U-Net that replaces the multigrid V-cycle
+ * (gpaw/preconditioner.py:9-93; used by rmmdiis.py:138-143): d = f(R, v).
+ * Activations: BF16 bits, channel-last (nbands, n0, n1, n2, C); periodic
+ * grids.  Convolution weights: BF16, packed by gh_unet_pack_conv3 from
+ * torch's conv3d layout W[co][ci][3][3][3] (gh_unet_conv3_weight_elems
+ * uint16 values); sums in FP32 on the matrix cores.
+ * gh_unet_conv3 mode 0 (first layer, cin = 2): channel 0 = scale[b] R[b],
+ * channel 1 = vscale v (R, v: dtype GH_F32 / GH_F64, band-major grids),
+ * staged as 4 channels (the last two zero): its weights are packed with
+ * gh_unet_pack_conv3(4, cout, W) from W[co][4][3][3][3];
+ * mode 1: act_in; mode 2: nearest 2x upsampling of act_in (the grid halved)
+ * plus skip.  The input slab of a block (3 x-planes, its rows plus halo)
+ * is staged in LDS once.  relu: apply max(0, .) after the bias.  Layers built:
+ * 2->8 (mode 0), 8->16, 16->32, 32->32, 8->8, 16->16 (mode 1), 16->16,
+ * 8->8 (mode 2). */
+int64_t gh_unet_conv3_weight_elems(int cin, int cout);
+int gh_unet_pack_conv3(int cin, int cout, const float* W, uint16_t* packed);
+int gh_unet_conv3(int cin, int cout, int mode, int dtype, int nbands,
+                  const int64_t* n_c, const void* in_dev, const void* skip_dev,
+                  const void* R_dev, const void* v_dev, const float* scale_dev,
+                  double vscale, const void* W_dev, const float* bias_dev,
+                  int relu, void* out_dev, void* stream);
+/* 2x2x2 average pooling (fine grid n_c, even), c = 8, 16 or 32. */
+int gh_unet_pool2(int c, int nbands, const int64_t* n_c, const void* in_dev,
+                  void* out_dev, void* stream);
+/* 1x1 channel map with FP32 weights W[ci][co] (32->16, 16->8). */
+int gh_unet_map1(int cin, int cout, int64_t npts, const void* in_dev,
+                 const float* W_dev, const float* bias_dev, void* out_dev,
+                 void* stream);
+/* out[b][p] = (sum_c W[c] act[b][p][c] + bias) / scale[b] (cin = 8; out of
+ * dtype GH_F32 or GH_F64). */
+int gh_unet_head(int cin, int dtype, int nbands, int64_t npts_per_band,
+                 const void* in_dev, const float* W_dev, double bias,
+                 const float* scale_dev, void* out_dev, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
