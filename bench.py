@@ -31,9 +31,10 @@ METRIC = ('band-Hψ/s at 112³ (GB/s vs 8 TB/s); RMM-DIIS time to '
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the fused stencil at the default workload (256
 # bands, 112^3, FP64), from rocprofv3 FETCH_SIZE / WRITE_SIZE runs of this
-# script scaled by a plain copy of known size (x2 / x1); the commands and
-# raw values are in profiles/full_hpsi_fp64_112cubed_256bands.md.
-STENCIL_TRAFFIC_BYTES = {(112, 256): 6.808e9}
+# script (--calibrate-copy) scaled by the plain copy of known size in the
+# same runs (x2.0000 / x1.0000); the commands and raw values are in
+# profiles/full_hpsi_fp64_112cubed_256bands.md.
+STENCIL_TRAFFIC_BYTES = {(112, 256): 6.820e9}
 
 
 def parse():
