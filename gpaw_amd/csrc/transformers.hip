@@ -19,6 +19,7 @@
 // registers), and keeps the last 2K-1 (restrict) or K (interpolate) filtered
 // planes in a register queue that the x-filter reads.
 #include <algorithm>
+#include <type_traits>
 #include <vector>
 
 #include "common.h"
@@ -45,6 +46,7 @@ struct TGeom {
     int zlo;       // ghost columns staged below z = 0
     int zw;        // staged row width: zlo + n[2] + ghost columns above
     int yz_trivial;  // Bloch phases along y and z are all 1 (k along x)
+    int vec;         // restriction: stage 16-byte vectors of vec elements
 };
 
 template <int K> struct W;
@@ -194,6 +196,82 @@ __device__ __forceinline__ void stage_store(const V* __restrict__ inb, int ix,
     }
 }
 
+// Staging in 16-byte vectors of VEC = 16 / sizeof(V) elements (8-byte and
+// 4-byte types): the staged row width and its ghost columns below z = 0 are
+// multiples of VEC and so is n2, so a vector never straddles a periodic
+// image or the row end, and its global address is 16-byte aligned.  One
+// load moves VEC elements (twice or four times the bytes in flight of the
+// element-wise staging); the phase of every element is applied at the
+// store as before.
+template <typename V, int VEC> struct VecT { V e[VEC]; };
+typedef float f4x __attribute__((ext_vector_type(4)));
+
+template <typename V, int VEC> struct StageRegsV {
+    static constexpr int SPT = 4;
+    static constexpr int kStageBatch = 4;
+    VecT<V, VEC> v[SPT];
+};
+
+template <typename V, typename S, int VEC>
+__device__ __forceinline__ VecT<V, VEC> vec_load(const V* __restrict__ inb, int r, int cv,
+                                                 int ix, int iy0, int nrows, const TGeom& g,
+                                                 const TPhase<S>& ph) {
+    VecT<V, VEC> v;
+    size_t off;
+    S pre, pim;
+    if (r < nrows && stage_elem(r, cv * VEC, ix, iy0, g, ph, off, pre, pim)) {
+        const f4x t = *reinterpret_cast<const f4x*>(inb + off);
+        __builtin_memcpy(&v, &t, 16);
+    } else {
+#pragma unroll
+        for (int e = 0; e < VEC; e++) v.e[e] = vzero<V>();
+    }
+    return v;
+}
+
+template <typename V, typename S, int VEC>
+__device__ __forceinline__ void vec_store(const VecT<V, VEC>& v, V* A, int r, int cv, int ix,
+                                          int iy0, const TGeom& g, const TPhase<S>& ph) {
+    VecT<V, VEC> o;
+#pragma unroll
+    for (int e = 0; e < VEC; e++)
+        o.e[e] = staged_value(v.e[e], r, cv * VEC + e, ix, iy0, g, ph);
+    f4x t;
+    __builtin_memcpy(&t, &o, 16);
+    *reinterpret_cast<f4x*>(A + r * g.zw + cv * VEC) = t;
+}
+
+template <typename V, typename S, int VEC>
+__device__ __forceinline__ void stage_load_v(const V* __restrict__ inb, int ix, int iy0,
+                                             int nrows, StageRegsV<V, VEC>& R,
+                                             const TGeom& g, const TPhase<S>& ph,
+                                             RowWalk w) {
+#pragma unroll
+    for (int s = 0; s < StageRegsV<V, VEC>::SPT; s++, w.next())
+        R.v[s] = vec_load<V, S, VEC>(inb, w.r, w.c, ix, iy0, nrows, g, ph);
+}
+
+template <typename V, typename S, int VEC>
+__device__ __forceinline__ void stage_store_v(const V* __restrict__ inb, int ix, int iy0,
+                                              int nrows, const StageRegsV<V, VEC>& R, V* A,
+                                              const TGeom& g, const TPhase<S>& ph,
+                                              RowWalk w) {
+#pragma unroll
+    for (int s = 0; s < StageRegsV<V, VEC>::SPT; s++, w.next())
+        if (w.r < nrows) vec_store<V, S, VEC>(R.v[s], A, w.r, w.c, ix, iy0, g, ph);
+    constexpr int NB = StageRegsV<V, VEC>::kStageBatch;
+    while (w.r < nrows) {
+        VecT<V, VEC> v[NB];
+        RowWalk w0 = w;
+#pragma unroll
+        for (int s = 0; s < NB; s++, w.next())
+            v[s] = vec_load<V, S, VEC>(inb, w.r, w.c, ix, iy0, nrows, g, ph);
+#pragma unroll
+        for (int s = 0; s < NB; s++, w0.next())
+            if (w0.r < nrows) vec_store<V, S, VEC>(v[s], A, w0.r, w0.c, ix, iy0, g, ph);
+    }
+}
+
 // Value of a staged row at z (row points at z = 0; the staged ghost
 // columns already carry their phase or zero).
 template <typename V, typename S>
@@ -217,7 +295,7 @@ __device__ __forceinline__ V restrict_tap(const V* row, int c, const TGeom& g,
 }
 
 // Restriction: output tile rows [J0, J0 + tj), output planes [I0, I1).
-template <typename V, typename S, int K, int PPT>
+template <typename V, typename S, int K, int PPT, int VEC = 1>
 __global__ void __launch_bounds__(1024)
 restrict_kernel(const V* __restrict__ in, V* __restrict__ out, TGeom g,
                 TPhase<S> ph, S alpha) {
@@ -225,7 +303,8 @@ restrict_kernel(const V* __restrict__ in, V* __restrict__ out, TGeom g,
     const int n2 = g.n[2], m2 = g.m[2];
     V* A = reinterpret_cast<V*>(smem_raw);   // nry x zw
     V* B = A + g.nry * g.zw;                 // nry x m2
-    const RowWalk wz(g.zw), wm(m2);          // thread's staged / filtered element
+    // thread's staged element (vector) and filtered element
+    const RowWalk wz(g.zw / VEC), wm(m2);
     const int per_band = g.nyt * g.nxc;
     const int b = xcd_swizzle(blockIdx.x, gridDim.x);
     const int band = b / per_band;
@@ -259,11 +338,16 @@ restrict_kernel(const V* __restrict__ in, V* __restrict__ out, TGeom g,
 
     // plane ix is in registers on entry (loaded during the previous
     // plane's filters); the next plane ix + 1 is loaded during this one's
-    StageRegs<V> pre;
+    typename std::conditional<(VEC > 1), StageRegsV<V, VEC>, StageRegs<V>>::type pre;
+    auto sload = [&](int ix) {
+        if constexpr (VEC > 1) stage_load_v<V, S, VEC>(inb, ix, iy0, nrows, pre, g, ph, wz);
+        else stage_load<V, S>(inb, ix, iy0, nrows, pre, g, ph, wz);
+    };
     auto process_plane = [&](int ix, int slot) {
-        stage_store<V, S>(inb, ix, iy0, nrows, pre, A, g, ph, wz);
+        if constexpr (VEC > 1) stage_store_v<V, S, VEC>(inb, ix, iy0, nrows, pre, A, g, ph, wz);
+        else stage_store<V, S>(inb, ix, iy0, nrows, pre, A, g, ph, wz);
         __syncthreads();
-        stage_load<V, S>(inb, ix + 1, iy0, nrows, pre, g, ph, wz);
+        sload(ix + 1);
         // z-filter every staged row
         RowWalk w = wm;
         for (; w.r < nrows; w.next()) {
@@ -290,7 +374,7 @@ restrict_kernel(const V* __restrict__ in, V* __restrict__ out, TGeom g,
     };
 
     // prime the first 2K-3 planes
-    stage_load<V, S>(inb, base, iy0, nrows, pre, g, ph, wz);
+    sload(base);
 #pragma unroll
     for (int s = 0; s < QL - 2; s++) process_plane(base + s, s);
     for (int I = I0; I < I1; I++) {
@@ -542,6 +626,11 @@ int launch_t(const gh_transformer* h, const void* in, void* out,
     if (h->interp)
         hipLaunchKernelGGL((interpolate_kernel<V, S, K, PPT_I>), dim3((unsigned)nblocks),
                            dim3(thr), lds, st, (const V*)in, (V*)out, g, ph, (S)alpha);
+    else if (constexpr int VEC = 16 / sizeof(V);
+             VEC > 1 && g.vec == VEC && ((uintptr_t)in & 15u) == 0)
+        hipLaunchKernelGGL((restrict_kernel<V, S, K, PPT_R, (VEC > 1 ? VEC : 1)>),
+                           dim3((unsigned)nblocks), dim3(thr), lds, st, (const V*)in,
+                           (V*)out, g, ph, (S)alpha);
     else
         hipLaunchKernelGGL((restrict_kernel<V, S, K, PPT_R>), dim3((unsigned)nblocks),
                            dim3(thr), lds, st, (const V*)in, (V*)out, g, ph, (S)alpha);
@@ -619,6 +708,19 @@ int gh_transformer_create(gh_transformer** out, const int64_t* nin_c,
         }
         h->g.zlo = zlo;
         h->g.zw = zlo + n2 + zhi;
+        // restriction of 8- and 4-byte types stages 16-byte vectors: the
+        // ghost columns are widened to whole vectors (within one period)
+        const int vec = 16 / dtype_size(dtype);
+        h->g.vec = 1;
+        if (!h->interp && vec > 1 && n2 % vec == 0) {
+            const int zl = (zlo + vec - 1) / vec * vec;
+            const int zh = (zhi + vec - 1) / vec * vec;
+            if (!h->g.pbc[2] || (zl <= n2 && zh <= n2)) {
+                h->g.zlo = zl;
+                h->g.zw = zl + n2 + zh;
+                h->g.vec = vec;
+            }
+        }
     }
     *out = h;
     return GH_OK;
