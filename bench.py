@@ -200,20 +200,18 @@ def cpu_baseline(w, gd_args, vt, nb_sample):
 
 def fp64_ritz_values(sys64, psit, dv):
     """Ritz values of the span of psit under the FP64 Hamiltonian: psit
-    promoted to FP64, H psi and the projections in FP64, S and H from
-    gh_gram (FP64), the generalized problem by Cholesky reduction."""
+    promoted to FP64, the full H psi (projector terms included) in FP64, S
+    and H from gh_gram (FP64), the generalized problem by Cholesky
+    reduction."""
     import torch
     from gpaw_amd import blas
     tdt = torch.complex128 if psit.is_complex() else torch.float64
     x = psit.to(tdt)
     hx = torch.empty_like(x)
-    P = sys64.H.apply(x, hx, sys64.phase_cd)
+    sys64.H.apply(x, hx, sys64.phase_cd)      # T + v + the projector terms
     f, hf = x.reshape(len(x), -1), hx.reshape(len(x), -1)
     Hm = blas.gram(f, hf, alpha=dv, hermitian=True, conj=True)
     Sm = blas.gram(f, alpha=dv, hermitian=True, conj=True)
-    if P is not None:
-        Hm = Hm + P.conj() @ sys64.H.dH(P).T
-        Hm = 0.5 * (Hm + Hm.conj().T)
     del x, hx
     L = torch.linalg.cholesky(Sm)
     Li = torch.linalg.solve_triangular(
