@@ -25,9 +25,9 @@ from .grid_descriptor import GridBoundsError
 class LocalizedFunctionsCollection:
     def __init__(self, gd, spline_aj, kd=None, cut=False, dtype=float,
                  integral=None, forces=None, xp=None):
-        if integral is not None:
-            raise NotImplementedError('normalised functions (integral=) are '
-                                      'not provided')
+        # integral: normalise on the grid (lfc.py:125-179) -- a number or
+        # one per atom; the first function of each atom must be l = 0
+        self.integral = integral
         self.gd = gd
         self.spline_aj = [list(s) for s in spline_aj]
         self.cut = cut
@@ -124,6 +124,8 @@ class LocalizedFunctionsCollection:
                 for k, (Mj0, A) in zip(keys, cols):
                     table[inv[off:off + len(k)], Mj0:Mj0 + A.shape[1]] = A
                     off += len(k)
+                if self.integral is not None:
+                    self._normalize(table, a, spline_j)
                 G_parts.append((ukeys % ngpts).astype(np.int32))
                 w_parts.append((ukeys // ngpts).astype(np.int32))
                 A_parts.append(table.ravel())
@@ -169,6 +171,24 @@ class LocalizedFunctionsCollection:
             float(self.gd.dv), dtype_code(self.dtype)))
         self.handle = h
         self.M_ptr = M_ptr
+
+    def _normalize(self, table, a, spline_j):
+        """Grid normalisation of one atom's functions (lfc.py:125-179): with
+        I_M = dv sum over the atom's points of A_M, the l = 0 function is
+        scaled to integrate to `integral` and every other function has that
+        scaled function times I_M / integral subtracted, so it integrates
+        to zero on the grid."""
+        integral = self.integral
+        if not np.isscalar(integral):
+            integral = integral[a]
+        if integral < 1e-15:
+            return
+        if spline_j[0].l != 0:
+            raise ValueError('normalisation needs an l = 0 first function')
+        I_m = self.gd.dv * table.sum(axis=0)
+        table[:, 0] *= integral / I_m[0]
+        if table.shape[1] > 1:
+            table[:, 1:] -= np.outer(table[:, 0], I_m[1:] / integral)
 
     # ---------------------------------------------------------- helpers --
     def get_function_count(self, a):
