@@ -25,6 +25,8 @@
 // contributions to a point is fixed (deterministic).
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
+#include <string>
 #include <numeric>
 #include <vector>
 
@@ -228,6 +230,116 @@ lfc_add_kernel(const V* __restrict__ c, V* __restrict__ a,
     }
 }
 
+// add, point-major: one read-modify-write per grid point of the spheres'
+// union (the slots, in grid order), all atoms in one launch:
+//   a[x, G_u] += sum_{entries e at u} sum_m c[x, M_e + m] A_e[m] conj(phase_e)
+// One wavefront per (chunk of 64 slots, band tile of BT bands); lane = slot,
+// so the loads and stores of a band run along the spheres' z-runs.  The
+// chunk's distinct atoms come from a host table; per atom the wave stages
+// the atom's coefficients c[x0 .. x0 + BT, M_a ..] in LDS (one coalesced
+// load), each lane folds its entries of that atom (several when a sphere
+// overlaps its own periodic image) into one coefficient row, and the band
+// sums read the staged coefficients as LDS broadcasts.  Every point is
+// written by one lane: no colouring, no atomics, and the order of the
+// contributions (atoms ascending) is fixed.
+// UNIF: every atom has MAXNM functions.
+template <typename V, typename S, int MAXNM, int BT, bool UNIF>
+__global__ void __launch_bounds__(64)
+lfc_add_points_kernel(const V* __restrict__ c, V* __restrict__ a,
+                      const int32_t* __restrict__ chunk_ptr,
+                      const int32_t* __restrict__ chunk_atom,
+                      const int32_t* __restrict__ slot_ptr,
+                      const int32_t* __restrict__ slot_G,
+                      const int32_t* __restrict__ se_atom,
+                      const int64_t* __restrict__ se_A,
+                      const int32_t* __restrict__ se_w,
+                      const int32_t* __restrict__ M_ptr, const S* __restrict__ A,
+                      const S* __restrict__ phase_w, int64_t nslots, int64_t ngpts,
+                      int Mmax, int nbands) {
+    constexpr bool CPLX = is_complex<V>::value;
+    __shared__ V cs[BT * MAXNM];
+    const int lane = threadIdx.x;
+    const int chunk = blockIdx.x;
+    const int64_t u = (int64_t)chunk * 64 + lane;
+    const bool live = u < nslots;
+    const int x0 = blockIdx.y * BT;
+    const int nb = min(BT, nbands - x0);
+    int p0 = 0, p1 = 0;
+    int64_t G = 0;
+    if (live) {
+        p0 = slot_ptr[u];
+        p1 = slot_ptr[u + 1];
+        G = slot_G[u];
+    }
+    V acc[BT];
+#pragma unroll
+    for (int b = 0; b < BT; b++) acc[b] = vzero<V>();
+    for (int k = chunk_ptr[chunk]; k < chunk_ptr[chunk + 1]; k++) {
+        const int at = chunk_atom[k];
+        const int M0 = M_ptr[at];
+        const int nm = UNIF ? MAXNM : M_ptr[at + 1] - M0;
+        // stage c[x0 + b, M0 + m] (b < nb, m < nm) at cs[b * MAXNM + m];
+        // one wave: its LDS writes and reads stay in program order
+        for (int i = lane; i < nb * MAXNM; i += 64) {
+            const int b = i / MAXNM, m = i - b * MAXNM;
+            if (UNIF || m < nm) cs[i] = c[(size_t)(x0 + b) * Mmax + M0 + m];
+        }
+        // this lane's coefficient row for atom `at` (zero without an entry;
+        // loads at m >= nm repeat the last function and weigh zero)
+        S are[MAXNM], aim[CPLX ? MAXNM : 1];
+#pragma unroll
+        for (int m = 0; m < MAXNM; m++) {
+            are[m] = 0;
+            if constexpr (CPLX) aim[m] = 0;
+        }
+        for (int p = p0; p < p1; p++) {
+            if (se_atom[p] != at) continue;
+            const S* __restrict__ Ae = A + se_A[p];
+            S pr = 1, pi = 0;
+            if (CPLX && phase_w) {
+                const int w = se_w[p];
+                if (w >= 0) { pr = phase_w[2 * w]; pi = -phase_w[2 * w + 1]; }
+            }
+            S v[MAXNM];
+#pragma unroll
+            for (int m = 0; m < MAXNM; m++) v[m] = Ae[UNIF || m < nm ? m : nm - 1];
+#pragma unroll
+            for (int m = 0; m < MAXNM; m++) {
+                const S vm = UNIF || m < nm ? v[m] : (S)0;
+                are[m] += vm * pr;
+                if constexpr (CPLX) aim[m] += vm * pi;
+            }
+        }
+#pragma unroll
+        for (int b = 0; b < BT; b++) {
+            if (b < nb) {
+#pragma unroll
+                for (int m = 0; m < MAXNM; m++) {
+                    if (!UNIF && m >= nm) break;
+                    const V cv = cs[b * MAXNM + m];
+                    if constexpr (CPLX) {
+                        acc[b].re += are[m] * cv.re - aim[m] * cv.im;
+                        acc[b].im += are[m] * cv.im + aim[m] * cv.re;
+                    } else {
+                        acc[b] += are[m] * cv;
+                    }
+                }
+            }
+        }
+    }
+    if (live) {
+        // all loads of the tile issued before its stores
+        V* __restrict__ ap = a + (size_t)x0 * ngpts + G;
+        V old[BT];
+#pragma unroll
+        for (int b = 0; b < BT; b++)
+            if (b < nb) old[b] = ap[(size_t)b * ngpts];
+#pragma unroll
+        for (int b = 0; b < BT; b++)
+            if (b < nb) ap[(size_t)b * ngpts] = vadd(old[b], acc[b]);
+    }
+}
+
 // out[x, M0+i] = sum_j P[x, M0+j] dH[j][i]  (- eps[x] sum_j P dS[j][i]),
 // summed in FP64 for every storage type V (W: its FP64 counterpart)
 template <typename V, typename W>
@@ -291,6 +403,18 @@ struct gh_lfc {
     int64_t* unit_end = nullptr;
     int32_t* color_units = nullptr;     // add units grouped by colour
     std::vector<int> color_ptr;         // host: colour c -> [ptr[c], ptr[c+1])
+    // point-major add: the slots (grid points of the spheres' union in grid
+    // order), their entries and each 64-slot chunk's distinct atoms
+    int64_t nslots = 0;
+    int nchunks = 0;
+    int32_t* slot_ptr = nullptr;        // slot -> its entries
+    int32_t* slot_G = nullptr;
+    int32_t* se_atom = nullptr;
+    int64_t* se_A = nullptr;
+    int32_t* se_w = nullptr;
+    int32_t* chunk_ptr = nullptr;       // chunk -> distinct atoms, ascending
+    int32_t* chunk_atom = nullptr;
+    bool points_add = true;             // GPAW_AMD_LFC_ADD=colour: colour launches
 };
 
 using namespace gh;
@@ -310,7 +434,8 @@ void free_lfc(gh_lfc* h) {
     if (!h) return;
     void* ptrs[] = {h->entry_ptr, h->M_ptr, h->A_off, h->dH_off, h->G_e, h->w_e,
                     h->A, h->phase_w, h->unit_atom, h->unit_beg, h->unit_end,
-                    h->color_units};
+                    h->color_units, h->chunk_ptr, h->chunk_atom, h->slot_ptr,
+                    h->slot_G, h->se_atom, h->se_A, h->se_w};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     delete h;
@@ -491,6 +616,55 @@ int gh_lfc_create(gh_lfc** out, int64_t ngpts, int natoms,
             h->phase_w = dp;
         }
     }
+    {
+        // point-major add tables: entries by grid point, for one point in
+        // entry order (atom, then layer); slots = the distinct points
+        std::vector<int64_t> order(ne);
+        std::iota(order.begin(), order.end(), 0);
+        std::stable_sort(order.begin(), order.end(),
+                         [&](int64_t x, int64_t y) { return G2[x] < G2[y]; });
+        std::vector<int32_t> atom_of(ne);
+        for (int a = 0; a < natoms; a++)
+            for (int64_t e = entry_ptr[a]; e < entry_ptr[a + 1]; e++) atom_of[e] = a;
+        std::vector<int32_t> slot_ptr(1, 0), slot_G, se_atom(ne), se_w(ne);
+        std::vector<int64_t> se_A(ne);
+        for (int64_t i = 0; i < ne; i++) {
+            const int64_t e = order[i];
+            const int a = atom_of[e];
+            se_atom[i] = a;
+            se_w[i] = w2[e];
+            se_A[i] = A_off[a] + (e - entry_ptr[a]) * (M_ptr[a + 1] - M_ptr[a]);
+            if (i == 0 || G2[e] != G2[order[i - 1]]) {
+                if (i > 0) slot_ptr.push_back((int32_t)i);
+                slot_G.push_back(G2[e]);
+            }
+        }
+        if (ne > 0) slot_ptr.push_back((int32_t)ne);
+        const int64_t ns = (int64_t)slot_G.size();
+        const int nch = (int)((ns + 63) / 64);
+        std::vector<int32_t> chunk_ptr(1, 0), chunk_atom;
+        for (int ch = 0; ch < nch; ch++) {
+            std::vector<int32_t> at;
+            const int64_t u1 = std::min<int64_t>(ns, (int64_t)(ch + 1) * 64);
+            for (int64_t u = (int64_t)ch * 64; u < u1; u++)
+                for (int p = slot_ptr[u]; p < slot_ptr[u + 1]; p++) at.push_back(se_atom[p]);
+            std::sort(at.begin(), at.end());
+            at.erase(std::unique(at.begin(), at.end()), at.end());
+            chunk_atom.insert(chunk_atom.end(), at.begin(), at.end());
+            chunk_ptr.push_back((int32_t)chunk_atom.size());
+        }
+        h->nslots = ns;
+        h->nchunks = nch;
+        UP(&h->slot_ptr, slot_ptr.data(), slot_ptr.size());
+        UP(&h->slot_G, slot_G.data(), slot_G.size());
+        UP(&h->se_atom, se_atom.data(), se_atom.size());
+        UP(&h->se_A, se_A.data(), se_A.size());
+        UP(&h->se_w, se_w.data(), se_w.size());
+        UP(&h->chunk_ptr, chunk_ptr.data(), chunk_ptr.size());
+        UP(&h->chunk_atom, chunk_atom.data(), chunk_atom.size());
+        const char* mode = getenv("GPAW_AMD_LFC_ADD");
+        h->points_add = !(mode && std::string(mode) == "colour");
+    }
 #undef UP
     *out = h;
     return GH_OK;
@@ -545,6 +719,30 @@ int gh_lfc_add(const gh_lfc* h, const void* c_dev, void* a_dev,
     if (nbands == 0 || h->nentries == 0) return GH_OK;
     hipStream_t st = (hipStream_t)stream;
     const void* ph = phase_table(h, q);
+    if (h->points_add && h->max_nm <= 32) {
+        const int mx = h->max_nm <= 8 ? 8 : h->max_nm <= 16 ? 16 : 32;
+#define PT_ARGS(V, S) (const V*)c_dev, (V*)a_dev, h->chunk_ptr, h->chunk_atom, h->slot_ptr, \
+                      h->slot_G, h->se_atom, h->se_A, h->se_w, h->M_ptr, (const S*)h->A,   \
+                      (const S*)ph, h->nslots, h->ngpts, h->Mmax, (int)nbands
+#define PT_LAUNCH(V, S, BT)                                                                   \
+    do {                                                                                      \
+        dim3 grid((unsigned)h->nchunks, (unsigned)((nbands + BT - 1) / BT));                  \
+        if (h->uniform_nm == 8) hipLaunchKernelGGL((lfc_add_points_kernel<V, S, 8, BT, true>), grid, dim3(64), 0, st, PT_ARGS(V, S)); \
+        else if (mx == 8) hipLaunchKernelGGL((lfc_add_points_kernel<V, S, 8, BT, false>), grid, dim3(64), 0, st, PT_ARGS(V, S)); \
+        else if (mx == 16) hipLaunchKernelGGL((lfc_add_points_kernel<V, S, 16, BT, false>), grid, dim3(64), 0, st, PT_ARGS(V, S)); \
+        else hipLaunchKernelGGL((lfc_add_points_kernel<V, S, 32, BT, false>), grid, dim3(64), 0, st, PT_ARGS(V, S)); \
+    } while (0)
+        switch (h->dtype) {
+            case GH_F64: PT_LAUNCH(double, double, 16); break;
+            case GH_C128: PT_LAUNCH(cplx<double>, double, 8); break;
+            case GH_F32: PT_LAUNCH(float, float, 32); break;
+            case GH_C64: PT_LAUNCH(cplx<float>, float, 16); break;
+        }
+#undef PT_LAUNCH
+#undef PT_ARGS
+        GH_CHECK_LAUNCH();
+        return GH_OK;
+    }
     const bool uniform8 = h->uniform_nm == 8;
     for (size_t col = 0; col + 1 < h->color_ptr.size(); col++) {
         const int n = h->color_ptr[col + 1] - h->color_ptr[col];
@@ -722,3 +920,4 @@ int64_t gh_spline_to_grid(int l, int nbins, double dr, const double* coefs,
 }
 
 }  // extern "C"
+

@@ -208,7 +208,12 @@ int gh_lfc_integrate(const gh_lfc* h, const void* a_dev, void* c_dev,
                      int64_t nbands, int q, void* stream);
 
 /* a[x, G] += sum_M c[x, M] A_M(G) [* conj(phase_q)]  (LFC.add_gpu,
- * lfc.py:416-419). */
+ * lfc.py:416-419).  One launch over the grid points of the spheres' union:
+ * one read-modify-write per point, the contributions of several atoms (or
+ * periodic images) summed in a fixed order (atoms ascending).  With
+ * GPAW_AMD_LFC_ADD=colour in the environment at gh_lfc_create (or more
+ * than 32 functions on one atom), per-atom launches instead, coloured so
+ * that no two launches' atoms share a point. */
 int gh_lfc_add(const gh_lfc* h, const void* c_dev, void* a_dev,
                int64_t nbands, int q, void* stream);
 

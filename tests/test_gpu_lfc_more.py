@@ -167,3 +167,45 @@ def test_shape_checks():
     with pytest.raises(ValueError):
         t.apply(a, torch.zeros((2, 6, 6, 6), dtype=torch.float64,
                                device='cuda'))
+
+
+@pytest.mark.parametrize('dtype,k_c', [(np.float64, None),
+                                       (np.complex128, (0.25, -0.2, 0.1)),
+                                       (np.float32, None),
+                                       (np.complex64, (0.25, 0.0, 0.0))])
+@pytest.mark.parametrize('ls_a,h', [
+    ([(0, 0, 1, 1)] * 7, 0.38),                 # 8 functions per atom
+    ([(0, 1, 2, 2), (0, 0, 1, 1), (2,)], 0.38),  # up to 14: 16-wide rows
+    ([(2, 2, 2, 2, 2), (0, 1)], 0.38),           # 25 functions: 32-wide rows
+    ([(0, 0, 1, 1)] * 3, 0.25),                 # spheres overlap their images
+])
+def test_points_add_equals_colour_add(monkeypatch, dtype, k_c, ls_a, h):
+    """The one-launch point-major add against the coloured per-atom
+    launches (GPAW_AMD_LFC_ADD=colour) and the oracle, with 37 bands
+    (ragged band tiles of both kernels)."""
+    rng = np.random.default_rng(25)
+    N_c = (16, 18, 20) if h > 0.3 else (10, 10, 12)
+    kd = None if k_c is None else np.array([k_c])
+    spos = rng.uniform(0, 1, (len(ls_a), 3))
+    spos[0] = (0.01, 0.98, 0.5)
+    lfc, oracle = build(np.random.default_rng(26), N_c, h, ls_a, dtype, kd,
+                        spos)
+    monkeypatch.setenv('GPAW_AMD_LFC_ADD', 'colour')
+    lfc_c, _ = build(np.random.default_rng(26), N_c, h, ls_a, dtype, kd,
+                     spos)
+    monkeypatch.delenv('GPAW_AMD_LFC_ADD')
+    a = rand(rng, (37,) + N_c, dtype)
+    coef = dev(rand(rng, (37, lfc.Mmax), dtype))
+    b_p = dev(a.copy())
+    b_c = dev(a.copy())
+    lfc.add_M(b_p, coef)
+    lfc_c.add_M(b_c, coef)
+    single = dtype in (np.float32, np.complex64)
+    assert relerr(host(b_p), host(b_c)) < (2e-6 if single else 1e-14)
+    wide = np.complex128 if np.dtype(dtype).kind == 'c' else np.float64
+    ref = oracle(k_c).add(a.astype(wide), host(coef).astype(wide))
+    assert relerr(host(b_p), ref) < (1e-5 if single else 1e-13)
+    # deterministic: the same bits twice
+    b_q = dev(a.copy())
+    lfc.add_M(b_q, coef)
+    assert torch.equal(b_p, b_q)
