@@ -77,3 +77,29 @@ def test_poisson_vs_oracle_and_fft(N_c):
     c, o = ops.operators[0]
     exact = opo.fft_solution(ogd, c, o, rho)
     assert abs(out - exact).max() < 1e-7 * abs(exact).max()
+
+
+@pytest.mark.parametrize('N_c', [(32, 32, 32), (24, 28, 20)])
+def test_poisson_graph_replays_equal_eager(N_c):
+    """The recorded V-cycle replayed (with the eager residual-norm read
+    back and synchronisation between replays) gives, cycle for cycle, the
+    residual norms and the potential of the eager solver, bit for bit; the
+    one recording serves later solves with other densities."""
+    rng = np.random.default_rng(4)
+    N_c = np.array(N_c)
+    ogd = og.Grid(N_c, N_c * 0.35)
+    gd = GridDescriptor(N_c, N_c * 0.35)
+    rhos = [charge(ogd, rng), charge(ogd, rng)]
+    ps_g = FDPoissonSolver(nn=3, eps=1e-14)
+    ps_g.set_grid_descriptor(gd)
+    ps_e = FDPoissonSolver(nn=3, eps=1e-14, graph=False)
+    ps_e.set_grid_descriptor(gd)
+    for rho in (rhos[0], rhos[1], rhos[0]):
+        phi_g = gd.zeros()
+        phi_e = gd.zeros()
+        n_g = ps_g.solve_neutral(phi_g, dev(rho))
+        n_e = ps_e.solve_neutral(phi_e, dev(rho))
+        assert n_g == n_e > 2
+        assert ps_g.errors == ps_e.errors
+        assert torch.equal(phi_g, phi_e)
+    assert ps_g._graph is not None
