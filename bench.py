@@ -223,7 +223,7 @@ def fp64_ritz_values(sys64, psit, dv):
 def eigensolver_run(w, gd, vt, nb, nconv, psi0, sweeps, dtype=np.float64,
                     k_c=None, solver='rmm-diis', tol=1e-8, polish=0,
                     polish_tol=1e-13, final_fp64=None, make_pc=None,
-                    sysm=None):
+                    sysm=None, warm=2):
     """RMM-DIIS (niter=3, all bands in one block) or Davidson (niter=2) with
     the multigrid preconditioner, wave functions and grid work in `dtype`
     (float64 / complex128: the plain FP64 form; float32 / complex64: the
@@ -234,7 +234,14 @@ def eigensolver_run(w, gd, vt, nb, nconv, psi0, sweeps, dtype=np.float64,
     at err <= tol.  The timed region runs from the first orthonormalisation
     to the sweep that reaches tol.  polish > 0: up to that many further
     sweeps (untimed) while the error exceeds polish_tol or still falls, for
-    the eigenvalues at the form's own floor."""
+    the eigenvalues at the form's own floor.
+
+    Before the timed run, `warm` sweeps on a copy of the start (a separate
+    solver object) load the GPU code objects and library handles this
+    precision uses for the first time in the process (FP32 / complex
+    GEMMs, eigh) and fill the allocator's cache; the timed run then
+    measures the solver's steady cost, as in an SCF loop.  The first,
+    cold, pass is timed too (`seconds_cold`, the same iterations)."""
     import torch
     from gpaw_amd import synthetic as syn
     from gpaw_amd._arrays import torch_dtype
@@ -242,18 +249,35 @@ def eigensolver_run(w, gd, vt, nb, nconv, psi0, sweeps, dtype=np.float64,
     if sysm is None:
         sysm = syn.System(w, gd, vt, dtype, k_c)
     pc = sysm.pc if make_pc is None else make_pc(sysm)
+    weights = torch.zeros(nb, dtype=torch.float64, device='cuda')
+    weights[:nconv] = 1.0
+    t_cold = None
+    if warm:
+        wfs = FDWaveFunctions(gd, psi0.to(torch_dtype(dtype)).contiguous(),
+                              sysm.pt, pc, sysm.phase_cd)
+        es = RMMDIIS() if solver == 'rmm-diis' else Davidson()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        wfs.orthonormalize(sysm.H)
+        for it in range(warm):
+            es.iterate(sysm.H, wfs, weights)
+        torch.cuda.synchronize()
+        t_cold = time.perf_counter() - t0
+        del wfs, es
+        if hasattr(pc, 'reset'):
+            pc.reset()
     wfs = FDWaveFunctions(gd, psi0.to(torch_dtype(dtype)).contiguous(),
                           sysm.pt, pc, sysm.phase_cd)
     es = RMMDIIS() if solver == 'rmm-diis' else Davidson()
-    weights = torch.zeros(nb, dtype=torch.float64, device='cuda')
-    weights[:nconv] = 1.0
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     wfs.orthonormalize(sysm.H)
-    errs, t_tol = [], None
+    errs, t_tol, t_first = [], None, None
     for it in range(sweeps):
         err = es.iterate(sysm.H, wfs, weights) / nconv
         errs.append(err)
+        if it == warm - 1:
+            t_first = time.perf_counter() - t0   # err is on the host: synced
         if err <= tol:
             torch.cuda.synchronize()
             t_tol = time.perf_counter() - t0
@@ -287,6 +311,13 @@ def eigensolver_run(w, gd, vt, nb, nconv, psi0, sweeps, dtype=np.float64,
            'errors': [float('%.3e' % e) for e in errs],
            'seconds_to_1e-8': None if t_tol is None else round(t_tol, 3),
            'eps_lowest': float(wfs.eps[0])}
+    if t_cold is not None:
+        # the same start and first sweeps in a process where this
+        # precision's kernels had not run yet
+        res['warm_sweeps'] = warm
+        res['seconds_cold_first_sweeps'] = round(t_cold, 3)
+        res['seconds_warm_first_sweeps'] = (None if t_first is None
+                                            else round(t_first, 3))
     if polish:
         res['polish_errors'] = [float('%.3e' % e) for e in perrs]
     if eps_rr is not None:

@@ -47,6 +47,13 @@ class LearnedPreconditioner:
     def calculate_kinetic_energy(self, psit_xG, kpt):
         return None
 
+    def reset(self):
+        """Forget the judge's decisions (a new run from a new start)."""
+        self.fallback_active = False
+        self.reject = None
+        self.calls = 0
+        self.rejected = 0
+
     def __call__(self, residuals, kpt=None, ekin=None, out=None):
         if self.fallback_active:
             return self.fallback(residuals, kpt, ekin, out)
