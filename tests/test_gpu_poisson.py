@@ -103,3 +103,115 @@ def test_poisson_graph_replays_equal_eager(N_c):
         assert ps_g.errors == ps_e.errors
         assert torch.equal(phi_g, phi_e)
     assert ps_g._graph is not None
+
+
+def _nonperiodic(n=40, h=0.3):
+    N_c = np.array((n, n, n))
+    gd = GridDescriptor(N_c, N_c * h, False)
+    ogd = og.Grid(N_c, N_c * h, False)
+    return gd, ogd
+
+
+def test_solve_charged_nonperiodic_monopole():
+    """A Gaussian charge at the centre of a non-periodic box: the monopole
+    Gaussian carries all of it, so phi is its free-space potential q v_0
+    (gpaw/poisson.py:227-277)."""
+    from gpaw_amd.poisson import Gaussian
+    gd, ogd = _nonperiodic()
+    g = Gaussian(gd)
+    Q = -1.7
+    rho = Q / np.sqrt(4 * np.pi) * g.get_gauss(0)
+    ps = FDPoissonSolver(nn=3, eps=1e-18)
+    ps.set_grid_descriptor(gd)
+    phi = gd.zeros()
+    niter = ps.solve(phi, dev(rho))
+    assert niter > 1
+    ref = Q / np.sqrt(4 * np.pi) * g.get_gauss_pot(0)
+    assert abs(host(phi) - ref).max() < 1e-9 * abs(ref).max()
+
+
+def test_solve_charged_nonperiodic_vs_oracle():
+    """Charged density of two Gaussians off the centre: the product equals
+    q v_0 + the oracle's neutral solve of rho - q g_0 from -q v_0."""
+    from gpaw_amd.poisson import Gaussian
+    gd, ogd = _nonperiodic(40, 0.3)
+    r = gd.get_grid_point_coordinates()
+    c0 = gd.cell_cv.sum(0) / 2
+    rho = np.zeros(tuple(gd.n_c))
+    for shift, w in (((1.0, 0.3, -0.5), 0.8), ((-0.7, -0.2, 0.9), 0.5)):
+        d2 = sum((r[i] - c0[i] - shift[i])**2 for i in range(3))
+        rho += w * np.exp(-d2 / 0.6)
+    g = Gaussian(gd)
+    Q = rho.sum() * gd.dv
+    q = Q / np.sqrt(4 * np.pi)
+    ps = FDPoissonSolver(nn=3, eps=1e-16)
+    ps.set_grid_descriptor(gd)
+    phi = gd.zeros()
+    ps.solve(phi, dev(rho))
+    ops = opo.FDPoissonSolver(nn=3, eps=1e-16)
+    ops.set_grid_descriptor(ogd)
+    ophi = -q * g.get_gauss_pot(0)
+    ops.solve_neutral(ophi, rho - q * g.get_gauss(0))
+    ref = ophi + q * g.get_gauss_pot(0)
+    assert abs(host(phi) - ref).max() < 1e-9 * abs(ref).max()
+
+
+def test_solve_remove_moment():
+    """remove_moment=4: the monopole and the three dipoles of a density
+    built from them are carried by Gaussians, leaving nothing for the
+    multigrid solve: phi = sum_L q_L v_L."""
+    from gpaw_amd.poisson import Gaussian
+    gd, ogd = _nonperiodic()
+    g = Gaussian(gd)
+    qs = [0.9, -0.3, 0.25, 0.4]
+    rho = sum(q * g.get_gauss(L) for L, q in enumerate(qs))
+    ps = FDPoissonSolver(nn=3, eps=1e-18, remove_moment=4)
+    ps.set_grid_descriptor(gd)
+    phi = gd.zeros()
+    ps.solve(phi, dev(rho))
+    ref = sum(q * g.get_gauss_pot(L) for L, q in enumerate(qs))
+    assert abs(host(phi) - ref).max() < 1e-8 * abs(ref).max()
+
+
+def test_solve_charged_periodic_background():
+    """A charged periodic cell: the uniform background of the net charge is
+    removed; against the exact (FFT) solution of the neutralised density."""
+    rng = np.random.default_rng(5)
+    N_c = np.array((24, 24, 24))
+    ogd = og.Grid(N_c, N_c * 0.35)
+    gd = GridDescriptor(N_c, N_c * 0.35)
+    rho = charge(ogd, rng) + 0.01          # net charge
+    assert abs(rho.sum() * gd.dv) > 1e-3
+    ps = FDPoissonSolver(nn=3, eps=1e-18)
+    ps.set_grid_descriptor(gd)
+    phi = gd.zeros()
+    ps.solve(phi, dev(rho))
+    ops = opo.FDPoissonSolver(nn=3)
+    ops.set_grid_descriptor(ogd)
+    c, o = ops.operators[0]
+    exact = opo.fft_solution(ogd, c, o, rho - rho.mean())
+    out = host(phi)
+    assert abs(out - exact).max() < 1e-8 * abs(exact).max()
+
+
+def test_solve_charge_center():
+    """use_charge_center: the monopole Gaussian sits at the centre of the
+    majority charge; for one off-centre Gaussian charge that is its own
+    centre, and phi approaches its free-space potential inside the box."""
+    from gpaw_amd.poisson import gaussian_potential
+    gd, ogd = _nonperiodic(48, 0.25)
+    r = gd.get_grid_point_coordinates()
+    r0 = gd.cell_cv.sum(0) / 2 + np.array((0.8, -0.5, 0.3))
+    xyz = [r[i] - r0[i] for i in range(3)]
+    r2 = sum(v * v for v in xyz)
+    a = 2.0
+    Q = 1.3
+    rho = Q * (a / np.pi)**1.5 * np.exp(-a * r2)
+    ps = FDPoissonSolver(nn=3, eps=1e-16, use_charge_center=True)
+    ps.set_grid_descriptor(gd)
+    phi = gd.zeros()
+    ps.solve(phi, dev(rho))
+    exact = Q / np.sqrt(4 * np.pi) * gaussian_potential(a, 0, *xyz, r2)
+    inner = r2 < 4.0
+    out = host(phi)
+    assert abs(out - exact)[inner].max() < 2e-4 * abs(exact).max()
