@@ -424,6 +424,133 @@ ring_tile(const V* __restrict__ in, V* __restrict__ out,
 
     // ------------------------------------------------------ compute --
     V* __restrict__ outb = out + band * bstride;
+    if constexpr (!is_complex<V>::value && PPT == 4) {
+        // FP32 data: each thread owns four z-adjacent points (one 16-byte
+        // vector) of one row (n2 % 4 == 0).  The same 1024 threads then
+        // cover twice the rows of the two-point path, so a tile carries
+        // fewer ghost rows per row, and every LDS read, potential / src
+        // load and store moves four points.  The z-window z - 4 .. z + 7
+        // is three vectors whose index wraps modulo n2 / 4 (R <= 4).
+        static_assert(R <= 4, "four-point path: R <= 4");
+        typedef V V4 __attribute__((ext_vector_type(4)));
+        const int nq = n2 / 4;                       // vectors per row
+        const int pidx = threadIdx.x;                // < ncomp * 64
+        const bool act = pidx < tye * nq;
+        const int pp = act ? pidx : 0;
+        const int yl = pp / nq;
+        const int zq = pp - yl * nq;                 // vector index in row
+        const int yz = (y0 + yl) * n2 + 4 * zq;
+        const int zqm = zq - 1, zqp = zq + 1;
+        const bool zout_m = zqm < 0 && !g.pbc2;
+        const bool zout_p = zqp >= nq && !g.pbc2;
+        const int zoff_m = zqm < 0 ? zqm + nq : zqm;
+        const int zoff_p = zqp >= nq ? zqp - nq : zqp;
+        if (!g.pbc1) {
+            for (int r = 0; r < 2 * R; r++) {
+                const int lrow = r < R ? r : tye + r;
+                const int y = y0 - R + lrow;
+                if (y >= 0 && y < g.n1) continue;
+                for (int sl = 0; sl < P; sl++)
+                    for (int e = pidx; e < n2; e += rg.ncomp * 64)
+                        ring[sl * slot + lrow * n2 + e] = (V)0;
+            }
+        }
+        V4 q[2 * R + 1];
+#pragma unroll
+        for (int j = 0; j < 2 * R; j++)
+            q[j] = V4{load_x(inb, x0 - R + j, yz, g, ph), load_x(inb, x0 - R + j, yz + 1, g, ph),
+                      load_x(inb, x0 - R + j, yz + 2, g, ph),
+                      load_x(inb, x0 - R + j, yz + 3, g, ph)};
+        S k0 = c0b;
+        S km[3][R], kp[3][R];
+        if (SYM) asm volatile("" : "+v"(k0));
+#pragma unroll
+        for (int c = 0; c < 3; c++)
+#pragma unroll
+            for (int j = 0; j < R; j++) {
+                km[c][j] = cf.cm[c][j];
+                kp[c][j] = SYM ? km[c][j] : cf.cp[c][j];
+                if (SYM) asm volatile("" : "+v"(km[c][j]));
+            }
+        auto march = [&](auto zper) {
+            constexpr bool ZPER = decltype(zper)::value;
+            for (int x = x0; x < x1; x++) {
+                ring_barrier();
+                {
+                    const int xp = x + R;
+                    const V4* sl = reinterpret_cast<const V4*>(ring + ((xp - x0) % P) * slot);
+                    V4 v = sl[(yl + R) * nq + zq];
+                    if (xp >= g.n0 && !g.pbc0) v = V4{0, 0, 0, 0};
+                    q[2 * R] = v;
+                }
+                const size_t gi = (size_t)x * plane + yz;
+                V4 vv, ss;
+                if (HAS_V) vv = *reinterpret_cast<const V4*>(vt + gi);
+                if (HAS_SRC) ss = *reinterpret_cast<const V4*>(src + band * bstride + gi);
+                const V4* sl = reinterpret_cast<const V4*>(ring + ((x - x0) % P) * slot);
+                V4 acc = q[R] * k0;
+#pragma unroll
+                for (int j = 0; j < R; j++) {
+                    if constexpr (SYM) {
+                        acc = (q[R - 1 - j] + q[R + 1 + j]) * km[0][j] + acc;
+                    } else {
+                        acc = q[R - 1 - j] * km[0][j] + acc;
+                        acc = q[R + 1 + j] * kp[0][j] + acc;
+                    }
+                }
+                const int row = yl + R;
+#pragma unroll
+                for (int j = 0; j < R; j++) {
+                    const V4 m = sl[(row - 1 - j) * nq + zq];
+                    const V4 pv = sl[(row + 1 + j) * nq + zq];
+                    if constexpr (SYM) {
+                        acc = (m + pv) * km[1][j] + acc;
+                    } else {
+                        acc = m * km[1][j] + acc;
+                        acc = pv * kp[1][j] + acc;
+                    }
+                }
+                // z-window: w[0] = vector zq - 1, w[1] = zq, w[2] = zq + 1
+                V4 w[3];
+                const V4* brow = sl + row * nq;
+                w[0] = brow[zoff_m];
+                w[1] = q[R];
+                w[2] = brow[zoff_p];
+                if constexpr (!ZPER) {
+                    if (zout_m) w[0] = V4{0, 0, 0, 0};
+                    if (zout_p) w[2] = V4{0, 0, 0, 0};
+                }
+                // element e of the window: vector e >> 2, lane e & 3; point
+                // i of this thread is element 4 + i
+                auto we = [&](int e) { return w[e >> 2][e & 3]; };
+                S a4[4] = {acc.x, acc.y, acc.z, acc.w};
+#pragma unroll
+                for (int i = 0; i < 4; i++)
+#pragma unroll
+                    for (int j = 0; j < R; j++) {
+                        if constexpr (SYM) {
+                            a4[i] = fma_rs(km[2][j], we(3 + i - j) + we(5 + i + j), a4[i]);
+                        } else {
+                            a4[i] = fma_rs(km[2][j], we(3 + i - j), a4[i]);
+                            a4[i] = fma_rs(kp[2][j], we(5 + i + j), a4[i]);
+                        }
+                    }
+                acc = V4{a4[0], a4[1], a4[2], a4[3]};
+                if (HAS_V) acc = vv * q[R] + acc;
+                if (HAS_SRC) acc = ss * beta + acc;
+                if (act) {
+                    V4* o = reinterpret_cast<V4*>(outb + gi);
+                    if (rg.nt_store) __builtin_nontemporal_store(acc, o);
+                    else *o = acc;
+                }
+#pragma unroll
+                for (int j = 0; j < 2 * R; j++) q[j] = q[j + 1];
+            }
+        };
+        if (g.pbc2) march(std::true_type{});
+        else march(std::false_type{});
+        return;
+    }
     if constexpr (!is_complex<V>::value && PPT == 2) {
         // Real data: each thread owns two z-adjacent points (z, z + 1) of
         // one row (n2 is even: rows are whole 16-byte multiples), so every
@@ -1253,7 +1380,7 @@ int launch_ring_r(const gh_stencil* h, const void* in, void* out,
     bool sym = true;
     for (int c = 0; c < 3; c++)
         for (int j = 0; j < R; j++) sym = sym && h->cm[c][j] == h->cp[c][j];
-    constexpr bool kPaired = PPT == 2;
+    constexpr bool kPaired = PPT == 2 || PPT == 4;   // the vector paths take SYM
 #define GH_LAUNCH_RING_S(HV, HS, SY)                                           \
     do {                                                                       \
         auto kfn = stencil_ring_kernel<V, S, R, PPT, HV, HS, SY>;              \
@@ -1284,7 +1411,7 @@ int launch_ring_r(const gh_stencil* h, const void* in, void* out,
 // points per compute thread of the ring kernel
 template <typename V> struct RingPPT { static constexpr int value = 2; };
 template <> struct RingPPT<cplx<double>> { static constexpr int value = 1; };
-template <> struct RingPPT<float> { static constexpr int value = 2; };
+template <> struct RingPPT<float> { static constexpr int value = 4; };
 template <> struct RingPPT<cplx<float>> { static constexpr int value = 2; };
 
 template <typename V, typename S, int PPT>
@@ -1296,7 +1423,7 @@ int launch_cfd(const gh_stencil* h, const void* in, void* out, int64_t nb,
         int rc = GH_ERR_UNSUPPORTED;
         // complex data takes the two-point path only when the y and z Bloch
         // phases are 1; otherwise the one-point path with phases
-        bool two = RP == 2;
+        bool two = RP > 1;
         if (is_complex<V>::value && ph)
             for (int c = 1; c < 3; c++)
                 for (int d = 0; d < 2; d++)
@@ -1387,7 +1514,7 @@ struct BatchPlan {
 template <typename V, typename S, int R>
 int plan_batch(BatchPlan& bp, const gh_stencil* const* ops, const int64_t* nbands,
                const int32_t* rows, const int32_t* depths) {
-    constexpr int PPT = 2;
+    constexpr int PPT = RingPPT<V>::value;
     const int vs = (int)sizeof(V);
     std::vector<RingCell<S>> cells(bp.ncells);
     std::vector<double> cost(bp.ncells);
@@ -1491,7 +1618,7 @@ int plan_batch(BatchPlan& bp, const gh_stencil* const* ops, const int64_t* nband
 template <typename V, typename S, int R>
 int launch_batch(const BatchPlan& bp, const void* const* in, void* const* out,
                  const void* const* vt, hipStream_t st) {
-    constexpr int PPT = 2;
+    constexpr int PPT = RingPPT<V>::value;
     BatchPtrs<V, S> p;
     for (int k = 0; k < kMaxBatchCells; k++) {
         p.in[k] = nullptr;
