@@ -477,8 +477,9 @@ def batched_leg():
     """Six independent cells of uneven size (40^3 .. 128^3, 16 .. 512 real
     bands, 8 .. 96 atoms; FP32 with FP64 reductions): the kinetic stencils
     with the local potential of all cells in one persistent launch (tiles
-    from the autotuner), then each cell's projections, P dH and projector
-    add; the whole step captured in a HIP graph.  Algorithmic bytes of the
+    from the batch autotuner), then each cell's projections, P dH and
+    projector add; the whole step captured in a HIP graph.  Compared with
+    one launch per cell, each with its own fastest tile.  Algorithmic bytes of the
     batched stencil: sum over cells of nbands * 2 * N * 4 + N * 4."""
     import torch
     from gpaw_amd import synthetic as syn
@@ -500,9 +501,13 @@ def batched_leg():
     def separate():
         for c, (w, gd, vt, nb, s) in enumerate(cells):
             s.H.apply_local(psis[c], outs[c])
+    from gpaw_amd.autotune import tune_stencil
+    own = [tune_stencil(s.kin, nb, with_v=True)
+           for w, gd, vt, nb, s in cells]       # each cell's fastest tile
+    t_sep = _time_ms(separate, reps=10)
     for c, (w, gd, vt, nb, s) in enumerate(cells):
         s.kin.set_tile(bh.batch.rows[c], 0, bh.batch.depths[c])
-    t_sep = _time_ms(separate, reps=10)
+    t_sep_same = _time_ms(separate, reps=10)
     t_eager = _time_ms(lambda: bh.apply(psis, outs), reps=10)
     bh.capture(psis, outs)
     t_graph = _time_ms(bh.replay, reps=10)
@@ -512,7 +517,9 @@ def batched_leg():
                        'tile_depth': bh.batch.depths[c]}
                       for c, (w, gd, vt, nb, s) in enumerate(cells)],
             'dtype': 'float32 (projections and P dH summed in FP64)',
-            'tiles_from': 'gpaw_amd.autotune.tune_stencil per cell',
+            'tiles_from': 'gpaw_amd.autotune.tune_batch: %s' % (
+                bh.tuned[3] if bh.tuned else 'default'),
+            'own_tiles': [[o[0], o[1]] if o else None for o in own],
             'launch': {'blocks': bh.batch.blocks,
                        'threads': bh.batch.threads,
                        'lds_bytes': bh.batch.lds, 'tiles': bh.batch.tiles,
@@ -522,6 +529,8 @@ def batched_leg():
             'stencil_batched_frac': round(alg / t_stencil / 1e6 /
                                           HBM_PEAK_GBS, 4),
             'stencil_one_launch_per_cell_ms': round(t_sep, 4),
+            'stencil_one_launch_per_cell_batch_tiles_ms': round(t_sep_same,
+                                                                4),
             'hpsi_step_eager_ms': round(t_eager, 4),
             'hpsi_step_graph_ms': round(t_graph, 4),
             'band_hpsi_per_s_graph': round(nbt / (t_graph * 1e-3), 1)}

@@ -3,8 +3,10 @@
 StencilBatch binds gh_stencil_batch: the kinetic stencils (with the local
 potential fused) of up to 8 cells -- each its own grid, band count and
 FDOperator -- run as one persistent kernel whose blocks walk a global tile
-list.  The tile of every cell (rows per tile, prefetch depth) comes from the
-autotuner (gpaw_amd.autotune.tune_stencil) unless given.
+list.  The tiles of the cells (rows per tile, prefetch depth) come from the
+batch autotuner (gpaw_amd.autotune.tune_batch: the batched launch timed for
+the cells' own fastest tiles and for the largest tiles each ring allows)
+unless given.
 
 BatchedHamiltonian applies H psi = T psi + v psi + sum_a p_a dH_a <p_a|psi>
 (gpaw/hamiltonian.py:445-479 per cell) to every cell: the one batched
@@ -96,22 +98,21 @@ class BatchedHamiltonian:
     grid, potential and projectors) with one stencil launch for all."""
 
     def __init__(self, hams, nbands, tune=True, tiles=None):
-        from .autotune import tune_stencil
+        from .autotune import tune_batch
         self.hams = list(hams)
         self.nbands = [int(n) for n in nbands]
-        rows, depths = [], []
-        self.tuned = []
-        for c, (H, nb) in enumerate(zip(self.hams, self.nbands)):
-            if tiles is not None:
-                r, d = tiles[c]
-            elif tune:
-                best = tune_stencil(H.kin, nb, with_v=True)
-                r, d = (best[0], best[1]) if best else (0, 0)
-                self.tuned.append(best)
-            else:
-                r, d = 0, 0
-            rows.append(r)
-            depths.append(d)
+        n = len(self.hams)
+        self.tuned = None
+        if tiles is not None:
+            rows = [r for r, d in tiles]
+            depths = [d for r, d in tiles]
+        elif tune:
+            best = tune_batch([H.kin for H in self.hams], self.nbands)
+            self.tuned = best
+            rows, depths = ((best[0], best[1]) if best
+                            else ([0] * n, [0] * n))
+        else:
+            rows, depths = [0] * n, [0] * n
         self.batch = StencilBatch([H.kin for H in self.hams], self.nbands,
                                   rows, depths, with_potential=True)
         self.graph = None
