@@ -77,3 +77,28 @@ def test_vcycle_256_bands(workload):
     c, o = ost.laplace_stencil(ogd.h_cv, -0.5, 3)
     ref = opc.Preconditioner(ogd, c, o)(psi[SAMPLES].cpu().numpy())
     assert relerr(d[SAMPLES].cpu().numpy(), ref) < 1e-12
+
+
+def test_single_precision_hpsi_and_vcycle_256_bands(workload):
+    """The FP32 forms at the benchmark's shape -- the four-point ring
+    stencil (28-row tiles), the FP32 point-major projector add with FP64
+    sums, the FP32 V-cycle -- against the FP64 oracle on sampled bands
+    (relative 2e-5 of the result's max-abs)."""
+    from gpaw_amd import synthetic as syn
+    w, gd, pt, kin, H, psi, out, oham, ogd = workload
+    s32 = syn.System(w, gd, oham.vt_G, np.float32)
+    x32 = psi[SAMPLES].float().contiguous()
+    # all 256 bands through the kernels, sampled bands compared
+    p32 = psi.float()
+    o32 = torch.empty_like(p32)
+    s32.H.apply(p32, o32)
+    torch.cuda.synchronize()
+    x = x32.double().cpu().numpy()
+    ref = oham.apply(x)
+    assert relerr(o32[SAMPLES].double().cpu().numpy(), ref) < 2e-5
+    d32 = s32.pc(p32, out=o32)
+    torch.cuda.synchronize()
+    c, o = ost.laplace_stencil(ogd.h_cv, -0.5, 3)
+    refd = opc.Preconditioner(ogd, c, o)(x)
+    assert relerr(d32[SAMPLES].double().cpu().numpy(), refd) < 2e-5
+    del p32, o32
