@@ -1086,6 +1086,50 @@ stencil_generic_kernel(const V* __restrict__ in, V* __restrict__ out,
     }
 }
 
+// Gauss-Seidel sweep (relax_method 1) for stencils whose offsets lie on
+// the grid axes: a point's neighbours before it in C order lie on earlier
+// hyperplanes i0 + i1 + i2 = const and those after it on later ones, so
+// updating the hyperplanes in order, each in one launch, reproduces the
+// reference's in-place C-order sweep; wrapped neighbours read `f0`, the
+// sweep's starting values (the reference fills its ghosts before the
+// sweep).  The coefficients are summed in stencil order.
+template <typename S>
+__global__ void __launch_bounds__(256)
+gs_plane_kernel(S* __restrict__ f, const S* __restrict__ f0, const S* __restrict__ src,
+                GenericArgs<S> a, int icentre, int plane, S inv) {
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    const int i0 = t / a.n1;
+    const int i1 = t - i0 * a.n1;
+    if (i0 >= a.n0) return;
+    const int i2 = plane - i0 - i1;
+    if (i2 < 0 || i2 >= a.n2) return;
+    S x = 0;
+    for (int p = 0; p < a.np; p++) {
+        if (p == icentre) continue;
+        int q0 = i0 + a.off[p][0], q1 = i1 + a.off[p][1], q2 = i2 + a.off[p][2];
+        const bool in = q0 >= 0 && q0 < a.n0 && q1 >= 0 && q1 < a.n1 && q2 >= 0 && q2 < a.n2;
+        S v;
+        if (in) {
+            v = f[((size_t)q0 * a.n1 + q1) * a.n2 + q2];
+        } else {
+            const bool ok = (q0 >= 0 && q0 < a.n0 ? true : a.pbc0) &&
+                            (q1 >= 0 && q1 < a.n1 ? true : a.pbc1) &&
+                            (q2 >= 0 && q2 < a.n2 ? true : a.pbc2);
+            if (ok) {
+                q0 = q0 < 0 ? q0 + a.n0 : q0 >= a.n0 ? q0 - a.n0 : q0;
+                q1 = q1 < 0 ? q1 + a.n1 : q1 >= a.n1 ? q1 - a.n1 : q1;
+                q2 = q2 < 0 ? q2 + a.n2 : q2 >= a.n2 ? q2 - a.n2 : q2;
+                v = f0[((size_t)q0 * a.n1 + q1) * a.n2 + q2];
+            } else {
+                v = 0;
+            }
+        }
+        x += v * a.coef[p];
+    }
+    const size_t i = ((size_t)i0 * a.n1 + i1) * a.n2 + i2;
+    f[i] = (src[i] - x) * inv;
+}
+
 }  // namespace gh
 
 // ------------------------------------------------------------- handle --
@@ -1764,13 +1808,51 @@ int gh_stencil_relax(const gh_stencil* h, int relax_method, void* f_dev,
     GH_REQUIRE(nrelax >= 0, GH_ERR_ARG, "relax: nrelax < 0");
     GH_REQUIRE(h->dtype == GH_F64 || h->dtype == GH_F32, GH_ERR_ARG,
                "relax: real grids only");
-    GH_REQUIRE(relax_method == 2, GH_ERR_UNSUPPORTED,
-               "relax: method %d not provided (only Jacobi, method 2; "
-               "Gauss-Seidel, method 1, is a sequential sweep)", relax_method);
+    GH_REQUIRE(relax_method == 1 || relax_method == 2, GH_ERR_ARG,
+               "relax: method %d (1 Gauss-Seidel, 2 Jacobi)", relax_method);
     GH_REQUIRE(h->diag != 0.0, GH_ERR_ARG, "relax: zero diagonal");
     GH_REQUIRE(f_dev != work_dev && s_dev != work_dev && f_dev != s_dev, GH_ERR_ARG,
                "relax: f, s and work must be distinct");
     hipStream_t st = (hipStream_t)stream;
+    if (relax_method == 1) {
+        GH_REQUIRE(h->cfd, GH_ERR_UNSUPPORTED,
+                   "relax: Gauss-Seidel needs a stencil along the grid axes");
+        GH_REQUIRE((int)h->coefs.size() <= kMaxGeneric, GH_ERR_UNSUPPORTED,
+                   "relax: too many stencil points");
+        const int n0 = (int)h->n[0], n1 = (int)h->n[1], n2 = (int)h->n[2];
+        int icentre = -1;
+        for (size_t p = 0; p < h->coefs.size(); p++)
+            if (h->offs[3 * p] == 0 && h->offs[3 * p + 1] == 0 && h->offs[3 * p + 2] == 0)
+                icentre = (int)p;
+        GH_REQUIRE(icentre >= 0, GH_ERR_ARG, "relax: stencil without a centre point");
+        const size_t bytes = (size_t)n0 * n1 * n2 * dtype_size(h->dtype);
+        const unsigned nblk = (unsigned)(((int64_t)n0 * n1 + 255) / 256);
+        const int nplanes = n0 + n1 + n2 - 2;
+#define GS_SWEEPS(S)                                                                   \
+        do {                                                                           \
+            GenericArgs<S> ga;                                                         \
+            ga.np = (int)h->coefs.size();                                              \
+            ga.n0 = n0; ga.n1 = n1; ga.n2 = n2;                                        \
+            ga.pbc0 = h->pbc[0]; ga.pbc1 = h->pbc[1]; ga.pbc2 = h->pbc[2];             \
+            for (int p = 0; p < ga.np; p++) {                                          \
+                ga.coef[p] = (S)h->coefs[p];                                           \
+                for (int c = 0; c < 3; c++) ga.off[p][c] = (signed char)h->offs[3 * p + c]; \
+            }                                                                          \
+            const S inv = (S)(1.0 / h->coefs[icentre]);                                \
+            for (int n = 0; n < nrelax; n++) {                                         \
+                GH_CHECK_HIP(hipMemcpyAsync(work_dev, f_dev, bytes, hipMemcpyDeviceToDevice, st)); \
+                for (int pl = 0; pl < nplanes; pl++)                                   \
+                    hipLaunchKernelGGL((gs_plane_kernel<S>), dim3(nblk), dim3(256), 0, st, \
+                                       (S*)f_dev, (const S*)work_dev, (const S*)s_dev, ga, \
+                                       icentre, pl, inv);                              \
+                GH_CHECK_LAUNCH();                                                     \
+            }                                                                          \
+        } while (0)
+        if (h->dtype == GH_F64) GS_SWEEPS(double);
+        else GS_SWEEPS(float);
+#undef GS_SWEEPS
+        return GH_OK;
+    }
     // f' = (1 - w) f + (w / c0) (s - (L f - c0 f)) = -(w / c0) L f + f + (w / c0) s
     const double a = -w / h->diag;
     const void* cur = f_dev;

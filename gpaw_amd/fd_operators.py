@@ -138,13 +138,16 @@ class FDOperator:
 
     def relax(self, relax_method, f_g, s_g, n, w=None):
         """n relaxation sweeps of L f = s in place on f (fd_operators.py:126,
-        c/operators.c:55-95).  relax_method 2 (Jacobi) with weight w
-        (default 1); method 1 (Gauss-Seidel, a sequential sweep) raises
-        NotImplementedError."""
-        if relax_method != 2:
+        c/operators.c:55-95, c/bmgs/relax.c).  relax_method 2: Jacobi with
+        weight w (default 1); relax_method 1: Gauss-Seidel in the
+        reference's C-order (w unused), run as one launch per hyperplane
+        i0 + i1 + i2 = const (stencils along the grid axes only)."""
+        if relax_method not in (1, 2):
+            raise ValueError('relax_method %r: 1 (Gauss-Seidel) or 2 '
+                             '(Jacobi)' % (relax_method,))
+        if relax_method == 1 and not self.cfd:
             raise NotImplementedError(
-                'relax_method %r: only Jacobi (2) is provided on the GPU'
-                % (relax_method,))
+                'Gauss-Seidel relaxation needs a stencil along the grid axes')
         assert tuple(f_g.shape) == self.shape == tuple(s_g.shape)
         assert not is_complex(self.dtype)
         w = 1.0 if w is None else float(w)
@@ -153,7 +156,7 @@ class FDOperator:
             if self._work is None or self._work.shape != f.shape:
                 self._work = torch.empty_like(f)
             _lib.check(self._lib.gh_stencil_relax(
-                self.handle, 2, f.data_ptr(), s.data_ptr(),
+                self.handle, int(relax_method), f.data_ptr(), s.data_ptr(),
                 self._work.data_ptr(), int(n), w, stream_ptr()))
 
     def get_diagonal_element(self):

@@ -10,8 +10,9 @@ sweep counts [2, 4, ...] / [1, 4, ...] (8 / 8 on the coarsest level) and
 weights [2/3, 1, ...].  iterate2 is the recursive V-cycle; solve_neutral
 repeats it until dv sum r^2 <= eps and removes the mean of phi for fully
 periodic cells; solve handles a net charge and removed multipoles
-(uniform background, Gaussian monopole, remove_moment).  Gauss-Seidel (relax='GS', a sequential sweep) is not
-provided.
+(uniform background, Gaussian monopole, remove_moment).  relax='GS'
+smooths with Gauss-Seidel sweeps in the reference's C order
+(FDOperator.relax method 1, one launch per hyperplane of the grid).
 
 With graph=True (the default) solve_neutral runs the first V-cycle
 eagerly, then records one V-cycle (every launch from the fine-level
@@ -138,9 +139,9 @@ class FDPoissonSolver:
                  metallic_electrodes=False,
                  use_charged_periodic_corrections=False, graph=True,
                  **kwargs):
-        if relax != 'J':
-            raise NotImplementedError(
-                'Relaxation method %s: only Jacobi (J) is provided' % relax)
+        if relax not in ('J', 'GS'):
+            raise ValueError('Relaxation method %s: J (Jacobi) or GS '
+                             '(Gauss-Seidel)' % relax)
         if metallic_electrodes:
             raise NotImplementedError('metallic_electrodes')
         if use_charged_periodic_corrections:
@@ -151,7 +152,7 @@ class FDPoissonSolver:
         self._gauss = None
         self.nn = nn
         self.relax = relax
-        self.relax_method = 2
+        self.relax_method = 1 if relax == 'GS' else 2
         self.eps = eps
         self.maxiter = maxiter
         self.use_graph = graph

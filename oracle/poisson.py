@@ -5,6 +5,12 @@ restatement (test infrastructure).
   c/bmgs/relax.c Jacobi branch): n sweeps of
   f <- (1 - w) f + w (s - sum_{c != 0} coef_c f[shift_c]) / coef_0
   with the right-hand side built from the previous sweep's f (padded copy).
+* relax_gauss_seidel: relax_method 1 (c/bmgs/relax.c:12-36): n sweeps
+  over the points in C order, each solving its own equation in place,
+  x = (s - sum_{c >= 1} coef_c a[p + offset_c]) / coef_0 with the
+  coefficients in stencil order; `a` is a padded copy whose ghosts are
+  filled from f before each sweep (periodic images or zero), so wrapped
+  neighbours keep the sweep's starting values.
 * FDPoissonSolver: gpaw/poisson.py:318-546 -- levels from repeated
   coarsening (at most 8), operators[0] = Laplace(gd, -1/(4 pi), nn) and
   7-point Laplacians on the coarse levels, nn=1 transformers, pre/post
@@ -13,8 +19,11 @@ restatement (test infrastructure).
   V-cycle; solve_neutral iterates until dv sum r^2 <= eps and removes the
   mean for fully periodic cells.
 
-Pinned by tests/test_oracle_poisson.py: relax_jacobi against the reference
-C (Operator.relax of oracle/_ref) and against a direct NumPy sweep; the
+Pinned by tests/test_oracle_poisson.py: relax_jacobi and
+relax_gauss_seidel against the reference C (Operator.relax of oracle/_ref),
+relax_jacobi against a direct NumPy sweep, relax_gauss_seidel by its
+defining property (each point's equation holds with its earlier
+neighbours' new values); the
 solver's converged potential against the exact solution of the same
 discrete equation from FFT (the stencil is diagonal in plane waves).
 """
@@ -35,8 +44,47 @@ def relax_jacobi(coefs, offsets, f, s, n, w, pbc_c):
     return f
 
 
+def relax_gauss_seidel(coefs, offsets, f, s, n, pbc_c):
+    """n Gauss-Seidel sweeps in place on f (one grid; small grids only:
+    a Python loop over the points)."""
+    offsets = np.asarray(offsets, int)
+    coefs = np.asarray(coefs, float)
+    i0 = np.flatnonzero((offsets == 0).all(1))[0]
+    others = [c for c in range(len(coefs)) if c != i0]
+    r = np.abs(offsets).max(0)
+    n_c = np.array(f.shape)
+    inv = 1.0 / coefs[i0]
+    for _ in range(n):
+        # padded copy, ghosts from the sweep's starting values
+        a = np.zeros(tuple(n_c + 2 * r))
+        a[r[0]:r[0] + n_c[0], r[1]:r[1] + n_c[1], r[2]:r[2] + n_c[2]] = f
+        for ax in range(3):
+            if pbc_c[ax] and r[ax] > 0:
+                sl = [slice(None)] * 3
+                src = [slice(None)] * 3
+                sl[ax] = slice(0, r[ax])
+                src[ax] = slice(n_c[ax], n_c[ax] + r[ax])
+                a[tuple(sl)] = a[tuple(src)]
+                sl[ax] = slice(n_c[ax] + r[ax], n_c[ax] + 2 * r[ax])
+                src[ax] = slice(r[ax], 2 * r[ax])
+                a[tuple(sl)] = a[tuple(src)]
+        for p0 in range(n_c[0]):
+            for p1 in range(n_c[1]):
+                for p2 in range(n_c[2]):
+                    q = (p0 + r[0], p1 + r[1], p2 + r[2])
+                    x = 0.0
+                    for c in others:
+                        o = offsets[c]
+                        x += a[q[0] + o[0], q[1] + o[1], q[2] + o[2]] * coefs[c]
+                    x = (s[p0, p1, p2] - x) * inv
+                    a[q] = x
+                    f[p0, p1, p2] = x
+    return f
+
+
 class FDPoissonSolver:
-    def __init__(self, nn=3, eps=2e-10, maxiter=1000):
+    def __init__(self, nn=3, eps=2e-10, maxiter=1000, relax='J'):
+        self.relax = relax
         self.nn = nn
         self.eps = eps
         self.maxiter = maxiter
@@ -77,8 +125,12 @@ class FDPoissonSolver:
 
     def _relax(self, level, n):
         c, o = self.operators[level]
-        relax_jacobi(c, o, self.phis[level], self.rhos[level], n,
-                     self.weights[level], self.gds[level].pbc_c)
+        if self.relax == 'GS':
+            relax_gauss_seidel(c, o, self.phis[level], self.rhos[level], n,
+                               self.gds[level].pbc_c)
+        else:
+            relax_jacobi(c, o, self.phis[level], self.rhos[level], n,
+                         self.weights[level], self.gds[level].pbc_c)
 
     def iterate2(self, level=0):
         if level < self.levels:

@@ -48,11 +48,51 @@ def test_relax_vs_oracle(nn, pbc, n):
     assert abs(host(fd) - ref).max() < 1e-12 * abs(ref).max()
 
 
-def test_relax_gauss_seidel_not_provided():
-    gd = GridDescriptor((8, 8, 8), (3.0, 3.0, 3.0))
-    f = torch.zeros((8, 8, 8), dtype=torch.float64, device='cuda')
-    with pytest.raises(NotImplementedError):
-        Laplace(gd, 1.0, 1).relax(1, f, f.clone(), 1)
+@pytest.mark.parametrize('nn,pbc,dtype', [(3, True, np.float64),
+                                          (1, True, np.float64),
+                                          (2, (True, False, True), np.float64),
+                                          (3, False, np.float32)])
+def test_relax_gauss_seidel_vs_oracle(nn, pbc, dtype):
+    """Gauss-Seidel sweeps (relax_method 1, one launch per hyperplane)
+    against the oracle's C-order sweep (itself pinned to the reference C)."""
+    from oracle.poisson import relax_gauss_seidel
+    rng = np.random.default_rng(6)
+    N_c = np.array([10, 12, 14])
+    gd = GridDescriptor(N_c, N_c * 0.3, pbc)
+    ogd = og.Grid(N_c, N_c * 0.3, pbc)
+    op = Laplace(gd, -0.25 / np.pi, nn, dtype)
+    f = rng.standard_normal(tuple(gd.n_c))
+    s = rng.standard_normal(tuple(gd.n_c))
+    fd = dev(f.astype(dtype))
+    op.relax(1, fd, dev(s.astype(dtype)), 3)
+    c, o = laplace_stencil(ogd.h_cv, -0.25 / np.pi, nn)
+    ref = relax_gauss_seidel(c, o, f.astype(dtype).astype(float), s.astype(
+        dtype).astype(float), 3, ogd.pbc_c)
+    tol = 2e-5 if dtype == np.float32 else 1e-13
+    assert abs(host(fd) - ref).max() < tol * abs(ref).max()
+
+
+def test_poisson_gauss_seidel_vs_oracle():
+    """FDPoissonSolver(relax='GS'): same V-cycle count and potential as
+    the oracle's solver with Gauss-Seidel smoothing."""
+    rng = np.random.default_rng(7)
+    N_c = np.array((16, 16, 16))
+    ogd = og.Grid(N_c, N_c * 0.35)
+    gd = GridDescriptor(N_c, N_c * 0.35)
+    rho = charge(ogd, rng)
+    ops = opo.FDPoissonSolver(nn=3, eps=1e-16, relax='GS')
+    ops.set_grid_descriptor(ogd)
+    ophi = ogd.zeros()
+    oniter = ops.solve_neutral(ophi, rho)
+    ps = FDPoissonSolver(nn=3, eps=1e-16, relax='GS')
+    ps.set_grid_descriptor(gd)
+    phi = gd.zeros()
+    niter = ps.solve_neutral(phi, dev(rho))
+    assert niter == oniter
+    assert abs(host(phi) - ophi).max() < 1e-10 * abs(ophi).max()
+    c, o = ops.operators[0]
+    exact = opo.fft_solution(ogd, c, o, rho)
+    assert abs(host(phi) - exact).max() < 1e-7 * abs(exact).max()
 
 
 @pytest.mark.parametrize('N_c', [(24, 24, 24), (24, 28, 20), (40, 40, 40)])

@@ -74,3 +74,44 @@ def test_poisson_solver_vs_fft(N):
     assert abs(phi - exact).max() < 1e-8 * abs(exact).max()
     # levels: coarsen while N is even (32 -> ... -> 1, 24 -> ... -> 3)
     assert ps.levels == {32: 5, 24: 3}[N]
+
+
+@pytest.mark.skipif(not refbuild.available(), reason='reference C not built')
+@pytest.mark.parametrize('nn,pbc', [(3, True), (1, True), (2, (True, False,
+                                                                 True))])
+def test_relax_gauss_seidel_vs_reference(nn, pbc):
+    rng = np.random.default_rng(3)
+    N_c = np.array([8, 10, 12])
+    gd = og.Grid(N_c, N_c * 0.3, pbc)
+    c, o = laplace_stencil(gd.h_cv, -0.25 / np.pi, nn)
+    f = rng.standard_normal(tuple(gd.n_c))
+    s = rng.standard_normal(tuple(gd.n_c))
+    op = refbuild.Operator(c, o, gd, True)
+    ref = f.copy()
+    op.relax(1, ref, s, 2, 1.0)
+    out = opo.relax_gauss_seidel(c, o, f.copy(), s, 2, gd.pbc_c)
+    assert abs(out - ref).max() < 1e-13 * abs(ref).max()
+
+
+def test_relax_gauss_seidel_defining_property():
+    """After one sweep every point's equation holds with the new values of
+    the neighbours before it in C order and the starting values of those
+    after it (and of every wrapped neighbour)."""
+    rng = np.random.default_rng(4)
+    N_c = np.array([6, 7, 8])
+    gd = og.Grid(N_c, N_c * 0.4)
+    c, o = laplace_stencil(gd.h_cv, 1.0, 2)
+    f0 = rng.standard_normal(tuple(N_c))
+    s = rng.standard_normal(tuple(N_c))
+    f1 = opo.relax_gauss_seidel(c, o, f0.copy(), s, 1, gd.pbc_c)
+    worst = 0.0
+    for p in np.ndindex(*N_c):
+        lhs = c[0] * f1[p]
+        for cc, oo in zip(c[1:], o[1:]):
+            q = np.array(p) + oo
+            wrapped = ((q < 0) | (q >= N_c)).any()
+            qm = tuple(q % N_c)
+            earlier = not wrapped and tuple(q) < p
+            lhs += cc * (f1[qm] if earlier else f0[qm])
+        worst = max(worst, abs(lhs - s[p]))
+    assert worst < 1e-12
