@@ -22,7 +22,7 @@ rmmdiis     Hamiltonian with dH/dS, subspace diagonalisation, inverse-Cholesky
             orthonormalisation and the blocked RMM-DIIS sweep
             (eigensolver.py:153-236, fdpw.py:394-424, rmmdiis.py:63-254)
 davidson    one Davidson iterate (eigensolvers/davidson.py:80-222)
-poisson     Jacobi relaxation and the multigrid Poisson solver
+poisson     Jacobi and Gauss-Seidel relaxation, the multigrid Poisson solver
             (c/bmgs/relax.c, c/operators.c:55-95, poisson.py:318-546)
 density     pseudo-density accumulation (c/utilities.c:313-356)
 refbuild    ctypes loader for ``oracle/_ref/libgpawref.so``, the reference C
