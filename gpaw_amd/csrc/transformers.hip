@@ -302,9 +302,9 @@ restrict_kernel(const V* __restrict__ in, V* __restrict__ out, TGeom g,
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int n2 = g.n[2], m2 = g.m[2];
     V* A = reinterpret_cast<V*>(smem_raw);   // nry x zw
-    V* B = A + g.nry * g.zw;                 // nry x m2
-    // thread's staged element (vector) and filtered element
-    const RowWalk wz(g.zw / VEC), wm(m2);
+    V* B = A + g.nry * g.zw;                 // tj x zw (y-filtered)
+    // thread's staged element (vector) and y-filtered element
+    const RowWalk wz(g.zw / VEC), wy(g.zw);
     const int per_band = g.nyt * g.nxc;
     const int b = xcd_swizzle(blockIdx.x, gridDim.x);
     const int band = b / per_band;
@@ -348,28 +348,29 @@ restrict_kernel(const V* __restrict__ in, V* __restrict__ out, TGeom g,
         else stage_store<V, S>(inb, ix, iy0, nrows, pre, A, g, ph, wz);
         __syncthreads();
         sload(ix + 1);
-        // z-filter every staged row
-        RowWalk w = wm;
-        for (; w.r < nrows; w.next()) {
-            const int r = w.r, kk = w.c;
-            int c = 2 * kk + (K - 1) - g.plo[2];
-            B[r * m2 + kk] = restrict_tap<V, S, K>(A + r * g.zw + g.zlo, c, g, ph);
+        // y-filter every staged column for the tile's output rows only
+        // (rows are already ghost-filled); filtering along z first would
+        // also filter the 2K - 3 rows that only feed the y taps
+        RowWalk w = wy;
+        for (; w.r < tje; w.next()) {
+            const int jr = w.r, col = w.c;
+            const int c = 2 * jr + (K - 1);
+            V acc = A[c * g.zw + col];
+#pragma unroll
+            for (int j = 0; j < K / 2; j++) {
+                const int o = 2 * j + 1;
+                acc = fma_rs((S)W<K>::w[j],
+                             vadd(A[(c + o) * g.zw + col], A[(c - o) * g.zw + col]), acc);
+            }
+            B[jr * g.zw + col] = vscale(acc, (S)0.5);
         }
         __syncthreads();
-        // y-filter (rows are already ghost-filled)
+        // z-filter at the own points
 #pragma unroll
         for (int k = 0; k < PPT; k++) {
             if (!act[k]) continue;
-            int c = 2 * jl[k] + (K - 1);
-            V acc = B[c * m2 + kz[k]];
-#pragma unroll
-            for (int j = 0; j < K / 2; j++) {
-                int o = 2 * j + 1;
-                acc = fma_rs((S)W<K>::w[j],
-                             vadd(B[(c + o) * m2 + kz[k]], B[(c - o) * m2 + kz[k]]),
-                             acc);
-            }
-            q[k][slot] = vscale(acc, (S)0.5);
+            const int c = 2 * kz[k] + (K - 1) - g.plo[2];
+            q[k][slot] = restrict_tap<V, S, K>(B + jl[k] * g.zw + g.zlo, c, g, ph);
         }
     };
 
@@ -577,7 +578,10 @@ int choose_rows(const gh_transformer* h, int vsize, int ppt, size_t* lds_out,
     const int maxrows = (!h->interp && h->nn == 1) ? 7 : g.m[1];
     for (int tj = 1; tj <= std::min(g.m[1], maxrows); tj++) {
         int nry = h->interp ? (tj + 1) / 2 + 2 * h->nn + 1 : 2 * tj + 2 * K - 3;
-        size_t lds = (size_t)nry * (g.zw + g.m[2]) * vsize;
+        // interpolation: staged rows + their z-filtered rows; restriction:
+        // staged rows + the tile's y-filtered rows (full staged width)
+        size_t lds = h->interp ? (size_t)nry * (g.zw + g.m[2]) * vsize
+                               : (size_t)(nry + tj) * g.zw * vsize;
         int thr = (tj * g.m[2] + ppt - 1) / ppt;
         if (lds > 64 * 1024 || thr > 1024) break;
         if (g.m[1] % tj == 0 || tj == g.m[1] || best == 0) {
