@@ -754,6 +754,20 @@ def main():
                                            8) / nb
     full_gbs = value * full_bytes_per_band / 1e9
 
+    # the achievable HBM rate: a plain streaming copy of the same array
+    # (16-byte vectors, non-temporal), best of a few grid sizes
+    from gpaw_amd import _lib
+    from gpaw_amd._arrays import stream_ptr
+    L = _lib.lib()
+    nbytes = psi.numel() * psi.element_size()
+    copy_gbs = {}
+    for nblk in (1024, 2048, 4096, 8192):
+        t = _time_ms(lambda: _lib.check(L.gh_stream_copy(
+            psi.data_ptr(), out.data_ptr(), nbytes, nblk, stream_ptr())),
+            reps=5)
+        copy_gbs[nblk] = 2 * nbytes / t / 1e6
+    copy_best = max(copy_gbs.values())
+
     c3 = None
     if not args.no_complex64:
         del out
@@ -821,7 +835,12 @@ def main():
                      'traffic': STENCIL_TRAFFIC_BYTES.get((N, nb)),
                      'kernel': 'stencil_ring_kernel<double,R=3,V> '
                                '(T psi + V psi, one pass)',
-                     'kernel_ms': round(stencil_ms, 4)},
+                     'kernel_ms': round(stencil_ms, 4),
+                     'traffic_GBps': (round(STENCIL_TRAFFIC_BYTES[(N, nb)] /
+                                            stencil_ms / 1e6, 1)
+                                      if (N, nb) in STENCIL_TRAFFIC_BYTES
+                                      else None),
+                     'copy_GBps': round(copy_best, 1)},
         'rmmdiis': rmm,
         'single_precision_complex_k': c3,
         'transformers': trans,

@@ -54,6 +54,7 @@ _SIGNATURES = {
     'gh_lfc_integrate': (_c.c_int, [P, P, P, I64, _c.c_int, P]),
     'gh_lfc_add': (_c.c_int, [P, P, P, I64, _c.c_int, P]),
     'gh_lfc_apply_dH': (_c.c_int, [P, P, P, P, P, P, I64, P]),
+    'gh_stream_copy': (_c.c_int, [P, P, I64, _c.c_int, P]),
     'gh_precond_scratch_bytes': (I64, [P, P, P, I64]),
     'gh_precond_apply': (_c.c_int, [P, P, P, P, P, P, P, D, P, P, I64, PD, P,
                                     P]),

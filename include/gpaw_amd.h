@@ -217,6 +217,13 @@ int gh_lfc_integrate(const gh_lfc* h, const void* a_dev, void* c_dev,
 int gh_lfc_add(const gh_lfc* h, const void* c_dev, void* a_dev,
                int64_t nbands, int q, void* stream);
 
+/* dst = src, nbytes (a multiple of 16, 16-byte aligned pointers) as a
+ * plain streaming copy in 16-byte vectors over nblocks x 256 threads
+ * (nblocks <= 0: 2048).  No counterpart in the reference: the measurement
+ * of the achievable HBM rate that bench.py reports next to the kernels. */
+int gh_stream_copy(const void* src_dev, void* dst_dev, int64_t nbytes, int nblocks,
+                   void* stream);
+
 /* --------------------------------------------- non-local Hamiltonian --
  * out[x, M_a + i] = sum_j P[x, M_a + j] dH_a[j][i] for every atom
  * (hamiltonian.py:23-44, apply_non_local_hamilton).  dH_dev holds the
