@@ -5,7 +5,8 @@ tiles of 32, integrate tiles of 16 bands, the LDS-ring stencil with band
 groups and the XCD-ordered block map, the V-cycle at 256 bands) compared
 with the oracle on sampled bands: H psi (gpaw/hamiltonian.py:445-479), the
 residual (H - eps S) psi (eigensolver.py:153-172) and the preconditioner
-(preconditioner.py:35-93)."""
+(preconditioner.py:35-93); and the single-precision and Bloch (k = (0.25, 0,
+0)) forms at the same shape."""
 import numpy as np
 import pytest
 
@@ -102,3 +103,42 @@ def test_single_precision_hpsi_and_vcycle_256_bands(workload):
     refd = opc.Preconditioner(ogd, c, o)(x)
     assert relerr(d32[SAMPLES].double().cpu().numpy(), refd) < 2e-5
     del p32, o32
+
+
+@pytest.mark.parametrize('dtype', [np.complex64, np.complex128])
+def test_complex_k_hpsi_and_vcycle_256_bands(workload, dtype):
+    """The Bloch (k = (0.25, 0, 0)) forms at the benchmark's shape: the
+    complex64 two-point ring path / complex128 one-point path with the
+    x-wrap phase, the projector sweeps with per-image phases, and the
+    V-cycle, against the complex128 oracle on sampled bands (2e-5 in
+    single precision, 1e-12 in double)."""
+    from gpaw_amd import synthetic as syn
+    w, gd, pt, kin, H, psi, out, oham, ogd = workload
+    k_c = (0.25, 0.0, 0.0)
+    sc = syn.System(w, gd, oham.vt_G, dtype, k_c)
+    td = torch.complex64 if dtype == np.complex64 else torch.complex128
+    g = torch.Generator(device='cuda')
+    g.manual_seed(5)
+    p = torch.randn((256, 112, 112, 112), dtype=td, device='cuda',
+                    generator=g)
+    o = torch.empty_like(p)
+    sc.H.apply(p, o, sc.phase_cd)
+    torch.cuda.synchronize()
+    x = p[SAMPLES].to(torch.complex128).cpu().numpy()
+    ph = ogd.phase_cd(k_c)
+    c, oo = ost.laplace_stencil(ogd.h_cv, -0.5, 3)
+    olf = olfc.LFC(ogd, [[osp.Spline(l, rc, f) for l, rc, f in pp]
+                         for pp in w.proj], k_c, complex)
+    olf.set_positions(w.spos_ac)
+    ok = orm.Hamiltonian(ogd, (c, oo), oham.vt_G, olf, w.dH, phase_cd=ph)
+    ref = ok.apply(x)
+    tol = 2e-5 if dtype == np.complex64 else 1e-12
+    assert relerr(o[SAMPLES].to(torch.complex128).cpu().numpy(), ref) < tol
+
+    class K:
+        phase_cd = sc.phase_cd
+    d = sc.pc(p, K, out=o)
+    torch.cuda.synchronize()
+    refd = opc.Preconditioner(ogd, c, oo)(x, ph)
+    assert relerr(d[SAMPLES].to(torch.complex128).cpu().numpy(), refd) < tol
+    del p, o
