@@ -1,7 +1,7 @@
 """Parity at the benchmark's own shapes (run on an MI355X): 256 real FP64
 bands on the 112^3 grid with 64 atoms x 8 projectors, built by
-bench.build_workload -- the launch shapes the benchmark times (LFC add band
-tiles of 32, integrate tiles of 16 bands, the LDS-ring stencil with band
+bench.build_workload -- the launch shapes the benchmark times (the
+point-major LFC add in 16-band tiles, integrate tiles of 16 bands, the LDS-ring stencil with band
 groups and the XCD-ordered block map, the V-cycle at 256 bands) compared
 with the oracle on sampled bands: H psi (gpaw/hamiltonian.py:445-479), the
 residual (H - eps S) psi (eigensolver.py:153-172) and the preconditioner
