@@ -112,8 +112,11 @@ def gram(a_xG, b_yG=None, alpha=1.0, hermitian=False, conj=False, out=None):
     if b_yG is None:
         b_yG = a_xG
     assert a_xG.dtype == b_yG.dtype and a_xG.is_cuda and b_yG.is_cuda
-    a = a_xG.reshape(a_xG.shape[0], -1) if a_xG.dim() > 1 else a_xG[None]
-    b = b_yG.reshape(b_yG.shape[0], -1) if b_yG.dim() > 1 else b_yG[None]
+    def rows(t):
+        if t.dim() == 1:
+            return t[None]
+        return t.reshape(t.shape[0], int(np.prod(t.shape[1:])))
+    a, b = rows(a_xG), rows(b_yG)
     assert a.shape[1] == b.shape[1], (a.shape, b.shape)
     assert a.stride(1) == 1 and b.stride(1) == 1
     m, n, k = a.shape[0], b.shape[0], a.shape[1]

@@ -427,10 +427,10 @@ int gh_stream_copy(const void* src_dev, void* dst_dev, int64_t nbytes, int nbloc
 
 int gh_multi_axpy(int dtype, int64_t nbands, int64_t n, const double* a_dev,
                   const void* x_dev, void* y_dev, void* stream) {
-    GH_REQUIRE(a_dev && x_dev && y_dev && n >= 0 && nbands >= 0, GH_ERR_ARG,
-               "multi_axpy: bad argument");
+    GH_REQUIRE(n >= 0 && nbands >= 0, GH_ERR_ARG, "multi_axpy: negative size");
     int64_t total = nbands * n;
-    if (total == 0) return GH_OK;
+    if (total == 0) return GH_OK;  // empty arrays may carry null pointers
+    GH_REQUIRE(a_dev && x_dev && y_dev, GH_ERR_ARG, "multi_axpy: null argument");
     hipStream_t st = (hipStream_t)stream;
     int64_t nblk = std::min<int64_t>((total + 255) / 256, 8192);
     switch (dtype) {
@@ -447,10 +447,10 @@ int gh_multi_axpy(int dtype, int64_t nbands, int64_t n, const double* a_dev,
 int gh_multi_dotc(int dtype, int64_t nbands, int64_t n, const void* a_dev,
                   const void* b_dev, double scale, double* out_dev,
                   double* work_dev, void* stream) {
-    GH_REQUIRE(a_dev && b_dev && out_dev && work_dev && n >= 0 && nbands >= 0,
-               GH_ERR_ARG, "multi_dotc: bad argument");
+    GH_REQUIRE(n >= 0 && nbands >= 0, GH_ERR_ARG, "multi_dotc: negative size");
     GH_REQUIRE(nbands < 65536, GH_ERR_ARG, "multi_dotc: too many bands per call");
     if (nbands == 0) return GH_OK;
+    GH_REQUIRE(a_dev && b_dev && out_dev && work_dev, GH_ERR_ARG, "multi_dotc: null argument");
     hipStream_t st = (hipStream_t)stream;
     dim3 grid(kDotChunks, (unsigned)nbands);
     switch (dtype) {
@@ -470,14 +470,14 @@ int gh_multi_dotc(int dtype, int64_t nbands, int64_t n, const void* a_dev,
 int gh_multi_lincomb(int dtype, int64_t nbands, int64_t n, int k,
                      const void* const* vecs_dev, const double* coef_dev,
                      void* out_dev, void* stream) {
-    GH_REQUIRE(vecs_dev && coef_dev && out_dev && n >= 0 && nbands >= 0, GH_ERR_ARG,
-               "multi_lincomb: bad argument");
+    GH_REQUIRE(n >= 0 && nbands >= 0, GH_ERR_ARG, "multi_lincomb: negative size");
     GH_REQUIRE(k >= 1 && k <= kLincombMax, GH_ERR_ARG,
                "multi_lincomb: k = %d out of range 1..%d", k, kLincombMax);
     GH_REQUIRE(nbands < 65536, GH_ERR_ARG, "multi_lincomb: too many bands per call");
+    if (nbands == 0 || n == 0) return GH_OK;  // empty arrays may carry null pointers
+    GH_REQUIRE(vecs_dev && coef_dev && out_dev, GH_ERR_ARG, "multi_lincomb: null argument");
     for (int a = 0; a < k; a++)
         GH_REQUIRE(vecs_dev[a], GH_ERR_ARG, "multi_lincomb: null vector %d", a);
-    if (nbands == 0 || n == 0) return GH_OK;
     hipStream_t st = (hipStream_t)stream;
     switch (dtype) {
         case GH_F64: return lincomb_k<double, double>(k, nbands, n, vecs_dev, coef_dev, out_dev, st);
@@ -497,14 +497,14 @@ int64_t gh_multi_gram_work(int k, int64_t nbands) {
 int gh_multi_gram(int dtype, int64_t nbands, int64_t n, int k,
                   const void* const* vecs_dev, double scale, double* out_dev,
                   double* work_dev, void* stream) {
-    GH_REQUIRE(vecs_dev && out_dev && work_dev && n >= 0 && nbands >= 0, GH_ERR_ARG,
-               "multi_gram: bad argument");
+    GH_REQUIRE(n >= 0 && nbands >= 0, GH_ERR_ARG, "multi_gram: negative size");
     GH_REQUIRE(k >= 1 && k <= kGramMax, GH_ERR_ARG, "multi_gram: k = %d out of range 1..%d",
                k, kGramMax);
     GH_REQUIRE(nbands < 65536, GH_ERR_ARG, "multi_gram: too many bands per call");
+    if (nbands == 0) return GH_OK;  // empty arrays may carry null pointers
+    GH_REQUIRE(vecs_dev && out_dev && work_dev, GH_ERR_ARG, "multi_gram: null argument");
     for (int a = 0; a < k; a++)
         GH_REQUIRE(vecs_dev[a], GH_ERR_ARG, "multi_gram: null vector %d", a);
-    if (nbands == 0) return GH_OK;
     hipStream_t st = (hipStream_t)stream;
     switch (dtype) {
         case GH_F64: return gram_k<double>(k, nbands, n, vecs_dev, scale, 0, out_dev, work_dev, st);
@@ -536,9 +536,9 @@ int gh_multi_gram_noalloc(int dtype, int64_t nbands, int64_t n, int k,
 
 int gh_add_to_density(int dtype, int64_t nbands, int64_t n, const double* f_dev,
                       const void* psit_dev, double* nt_dev, void* stream) {
-    GH_REQUIRE(f_dev && psit_dev && nt_dev && n >= 0 && nbands >= 0, GH_ERR_ARG,
-               "add_to_density: bad argument");
-    if (n == 0 || nbands == 0) return GH_OK;
+    GH_REQUIRE(n >= 0 && nbands >= 0, GH_ERR_ARG, "add_to_density: negative size");
+    if (n == 0 || nbands == 0) return GH_OK;  // empty arrays may carry null pointers
+    GH_REQUIRE(f_dev && psit_dev && nt_dev, GH_ERR_ARG, "add_to_density: null argument");
     hipStream_t st = (hipStream_t)stream;
     const int64_t nblk = std::min<int64_t>((n + 255) / 256, 16384);
     switch (dtype) {
@@ -567,10 +567,14 @@ int gh_precond_apply(const gh_stencil* kin0, const gh_stencil* kin1,
                      const gh_transformer* intp1, double step,
                      const void* R_dev, void* d0_dev, int64_t nbands,
                      const double* phase_cd, void* scratch_dev, void* stream) {
-    GH_REQUIRE(kin0 && kin1 && kin2 && rest0 && rest1 && intp2 && intp1 &&
-                   R_dev && d0_dev && scratch_dev,
-               GH_ERR_ARG, "precond_apply: null argument");
-    GH_REQUIRE(R_dev != d0_dev, GH_ERR_ARG, "precond_apply: R and d0 must differ");
+    GH_REQUIRE(kin0 && kin1 && kin2 && rest0 && rest1 && intp2 && intp1, GH_ERR_ARG,
+               "precond_apply: null handle");
+    GH_REQUIRE(nbands >= 0, GH_ERR_ARG, "precond_apply: nbands < 0");
+    // empty arrays may carry null pointers
+    GH_REQUIRE(nbands == 0 || (R_dev && d0_dev && scratch_dev), GH_ERR_ARG,
+               "precond_apply: null argument");
+    GH_REQUIRE(nbands == 0 || R_dev != d0_dev, GH_ERR_ARG,
+               "precond_apply: R and d0 must differ");
     const int dt = stencil_dtype(kin0);
     GH_REQUIRE(stencil_dtype(kin1) == dt && stencil_dtype(kin2) == dt, GH_ERR_ARG,
                "precond_apply: dtype mismatch");

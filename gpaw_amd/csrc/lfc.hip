@@ -686,10 +686,12 @@ static const void* phase_table(const gh_lfc* h, int q) {
 
 int gh_lfc_integrate(const gh_lfc* h, const void* a_dev, void* c_dev,
                      int64_t nbands, int q, void* stream) {
-    GH_REQUIRE(h && a_dev && c_dev, GH_ERR_ARG, "lfc_integrate: null argument");
+    GH_REQUIRE(h, GH_ERR_ARG, "lfc_integrate: null handle");
+    GH_REQUIRE(nbands >= 0, GH_ERR_ARG, "lfc_integrate: nbands < 0");
     GH_REQUIRE(q >= 0 && (q < h->nq || (h->nq == 0 && q == 0)), GH_ERR_ARG,
                "lfc_integrate: k-point index %d out of range (%d tables)", q, h->nq);
-    if (nbands == 0 || h->natoms == 0) return GH_OK;
+    if (nbands == 0 || h->natoms == 0) return GH_OK;  // nothing to read or write
+    GH_REQUIRE(a_dev && c_dev, GH_ERR_ARG, "lfc_integrate: null argument");
     // every (band, M) is written by the block of its atom, also for atoms
     // without grid points, so no clearing pass is needed
     hipStream_t st = (hipStream_t)stream;
@@ -713,10 +715,12 @@ int gh_lfc_integrate(const gh_lfc* h, const void* a_dev, void* c_dev,
 
 int gh_lfc_add(const gh_lfc* h, const void* c_dev, void* a_dev,
                int64_t nbands, int q, void* stream) {
-    GH_REQUIRE(h && a_dev && c_dev, GH_ERR_ARG, "lfc_add: null argument");
+    GH_REQUIRE(h, GH_ERR_ARG, "lfc_add: null handle");
+    GH_REQUIRE(nbands >= 0, GH_ERR_ARG, "lfc_add: nbands < 0");
     GH_REQUIRE(q >= 0 && (q < h->nq || (h->nq == 0 && q == 0)), GH_ERR_ARG,
                "lfc_add: k-point index %d out of range (%d tables)", q, h->nq);
-    if (nbands == 0 || h->nentries == 0) return GH_OK;
+    if (nbands == 0 || h->nentries == 0) return GH_OK;  // nothing to read or write
+    GH_REQUIRE(a_dev && c_dev, GH_ERR_ARG, "lfc_add: null argument");
     hipStream_t st = (hipStream_t)stream;
     const void* ph = phase_table(h, q);
     if (h->points_add && h->max_nm <= 32) {
@@ -786,9 +790,11 @@ int gh_lfc_add(const gh_lfc* h, const void* c_dev, void* a_dev,
 int gh_lfc_apply_dH(const gh_lfc* h, const void* P_dev, const double* dH_dev,
                     const double* eps_dev, const double* dS_dev,
                     void* out_dev, int64_t nbands, void* stream) {
-    GH_REQUIRE(h && P_dev && dH_dev && out_dev, GH_ERR_ARG, "lfc_apply_dH: null argument");
+    GH_REQUIRE(h, GH_ERR_ARG, "lfc_apply_dH: null handle");
+    GH_REQUIRE(nbands >= 0, GH_ERR_ARG, "lfc_apply_dH: nbands < 0");
+    if (nbands == 0 || h->Mmax == 0) return GH_OK;  // nothing to read or write
+    GH_REQUIRE(P_dev && dH_dev && out_dev, GH_ERR_ARG, "lfc_apply_dH: null argument");
     GH_REQUIRE(P_dev != out_dev, GH_ERR_ARG, "lfc_apply_dH: P and out must differ");
-    if (nbands == 0 || h->Mmax == 0) return GH_OK;
     hipStream_t st = (hipStream_t)stream;
     dim3 grid((h->Mmax + 127) / 128, (unsigned)nbands);
 #define DH_ARGS(V) (const V*)P_dev, dH_dev, h->dH_off, eps_dev, dS_dev, h->M_ptr, \

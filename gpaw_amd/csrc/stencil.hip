@@ -1795,7 +1795,7 @@ int gh_stencil_apply_shifted(const gh_stencil* h, const void* in_dev,
                              void* out_dev, int64_t nbands,
                              const double* phase_cd, const void* vt_dev,
                              const double* eps_dev, void* stream) {
-    GH_REQUIRE(eps_dev, GH_ERR_ARG, "stencil_apply_shifted: null eps");
+    GH_REQUIRE(eps_dev || nbands == 0, GH_ERR_ARG, "stencil_apply_shifted: null eps");
     return gh::stencil_apply_full(h, in_dev, out_dev, nbands, phase_cd, vt_dev,
                                   nullptr, 0.0, 1.0, 0.0, eps_dev,
                                   (hipStream_t)stream);
@@ -1991,10 +1991,11 @@ int stencil_apply_full(const gh_stencil* h, const void* in_dev, void* out_dev,
                        int64_t nbands, const double* phase_cd,
                        const void* vt_dev, const void* src_dev, double beta,
                        double al, double ga, const double* eps, hipStream_t st) {
-    GH_REQUIRE(h && in_dev && out_dev, GH_ERR_ARG, "stencil_apply: null argument");
-    GH_REQUIRE(in_dev != out_dev, GH_ERR_ARG, "stencil_apply: in and out must differ");
+    GH_REQUIRE(h, GH_ERR_ARG, "stencil_apply: null handle");
     GH_REQUIRE(nbands >= 0, GH_ERR_ARG, "stencil_apply: nbands < 0");
-    if (nbands == 0) return GH_OK;
+    if (nbands == 0) return GH_OK;  // empty arrays may carry null pointers
+    GH_REQUIRE(in_dev && out_dev, GH_ERR_ARG, "stencil_apply: null argument");
+    GH_REQUIRE(in_dev != out_dev, GH_ERR_ARG, "stencil_apply: in and out must differ");
     if (h->cfd) {
         switch (h->dtype) {
             case GH_F64: return launch_cfd<double, double, 4>(h, in_dev, out_dev, nbands, phase_cd, vt_dev, src_dev, beta, al, ga, eps, st);

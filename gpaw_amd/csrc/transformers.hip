@@ -757,10 +757,11 @@ int transformer_apply_scaled(const gh_transformer* h, const void* in_dev,
                              void* out_dev, int64_t nbands,
                              const double* phase_cd, double al,
                              hipStream_t st) {
-    GH_REQUIRE(h && in_dev && out_dev, GH_ERR_ARG, "transformer_apply: null argument");
-    GH_REQUIRE(in_dev != out_dev, GH_ERR_ARG, "transformer_apply: in and out must differ");
+    GH_REQUIRE(h, GH_ERR_ARG, "transformer_apply: null handle");
     GH_REQUIRE(nbands >= 0, GH_ERR_ARG, "transformer_apply: nbands < 0");
-    if (nbands == 0) return GH_OK;
+    if (nbands == 0) return GH_OK;  // empty arrays may carry null pointers
+    GH_REQUIRE(in_dev && out_dev, GH_ERR_ARG, "transformer_apply: null argument");
+    GH_REQUIRE(in_dev != out_dev, GH_ERR_ARG, "transformer_apply: in and out must differ");
     switch (h->dtype) {
         case GH_F64: return launch_k<double, double>(h, in_dev, out_dev, nbands, phase_cd, al, st);
         case GH_C128: return launch_k<cplx<double>, double>(h, in_dev, out_dev, nbands, phase_cd, al, st);

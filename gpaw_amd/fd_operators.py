@@ -116,8 +116,9 @@ class FDOperator:
         if not self.cfd:
             from . import blas
             self.apply(in_xg, out_xg, phase_cd, vt_g=vt_g)
-            blas.multi_axpy(-eps, in_xg.reshape(nb, -1),
-                            out_xg.reshape(nb, -1))
+            gsize = int(np.prod(self.shape))
+            blas.multi_axpy(-eps, in_xg.reshape(nb, gsize),
+                            out_xg.reshape(nb, gsize))
             return
         ph, php = _lib.phase_arg(phase_cd)
         vt = None

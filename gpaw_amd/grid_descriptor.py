@@ -101,13 +101,13 @@ class GridDescriptor:
         a = a_xg if isinstance(a_xg, torch.Tensor) else torch.from_numpy(
             np.ascontiguousarray(a_xg)).cuda()
         xshape = tuple(a.shape[:-3])
+        gsize = int(np.prod(a.shape[-3:]))
         if b_yg is None:
-            result = a.reshape(xshape + (-1,)).sum(dim=-1) * self.dv
+            result = a.reshape(xshape + (gsize,)).sum(dim=-1) * self.dv
             return result.item() if result.ndim == 0 else result
         same = b_yg is a_xg
         b = a if same else (b_yg if isinstance(b_yg, torch.Tensor) else
                             torch.from_numpy(np.ascontiguousarray(b_yg)).cuda())
-        gsize = int(np.prod(a.shape[-3:]))
         A = a.reshape((-1, gsize)).contiguous()
         B = b.reshape((-1, gsize)).contiguous()
         mdt = torch.complex128 if A.is_complex() else torch.float64

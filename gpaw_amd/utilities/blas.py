@@ -35,7 +35,16 @@ def _writeback(orig, t, staged):
 
 
 def _flat2(a):
-    return a.reshape(a.shape[0], -1)
+    # explicit sizes: torch cannot infer a -1 axis of an empty array
+    return a.reshape(a.shape[0], int(np.prod(a.shape[1:])))
+
+
+def _first_two_flat(a):
+    """(n0, n1, rest) view, then the first two axes swapped and the other
+    ones combined: (n1, n0 * rest)."""
+    rest = int(np.prod(a.shape[2:]))
+    a3 = a.reshape(a.shape[0], a.shape[1], rest).permute(1, 0, 2)
+    return a3.reshape(a.shape[1], a.shape[0] * rest)
 
 
 def _tall(a2, b2):
@@ -94,8 +103,7 @@ def rk(alpha, a, beta, c, trans='c'):
             prod = a2 @ a2.conj().T
     else:
         assert tuple(c.shape) == (a.shape[1], a.shape[1])
-        a2 = ta.reshape(ta.shape[0], ta.shape[1], -1)
-        a2 = a2.permute(1, 0, 2).reshape(ta.shape[1], -1)
+        a2 = _first_two_flat(ta)
         prod = (a2.conj() @ a2.T)
     prod = alpha * prod
     if beta == 0.0:
@@ -125,10 +133,7 @@ def r2k(alpha, a, b, beta, c, trans='c'):
             G = a2 @ b2.conj().T
     else:
         assert tuple(c.shape) == (a.shape[1], a.shape[1])
-        a2 = ta.reshape(ta.shape[0], ta.shape[1], -1).permute(1, 0, 2)
-        b2 = tb.reshape(tb.shape[0], tb.shape[1], -1).permute(1, 0, 2)
-        a2 = a2.reshape(ta.shape[1], -1)
-        b2 = b2.reshape(tb.shape[1], -1)
+        a2, b2 = _first_two_flat(ta), _first_two_flat(tb)
         G = a2.conj() @ b2.T                          # a^H b
     # c/blas.c:140-170 hands row-major arrays to zher2k as their Fortran
     # transposes: what it forms is alpha G^H + conj(alpha) G

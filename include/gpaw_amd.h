@@ -15,6 +15,9 @@
  * default stream) and is asynchronous unless stated otherwise.  Handles are
  * immutable after creation and may be used from several streams at once;
  * scratch memory is owned by the caller or allocated per handle as stated.
+ * A call with nbands == 0 (an empty band set, as the reference's loops over
+ * the leading axis allow) returns GH_OK without touching memory; its array
+ * pointers may then be NULL, as empty device arrays' pointers are.
  */
 #ifndef GPAW_AMD_H
 #define GPAW_AMD_H
