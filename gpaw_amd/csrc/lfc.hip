@@ -793,6 +793,7 @@ int gh_lfc_apply_dH(const gh_lfc* h, const void* P_dev, const double* dH_dev,
     GH_REQUIRE(h, GH_ERR_ARG, "lfc_apply_dH: null handle");
     GH_REQUIRE(nbands >= 0, GH_ERR_ARG, "lfc_apply_dH: nbands < 0");
     if (nbands == 0 || h->Mmax == 0) return GH_OK;  // nothing to read or write
+    GH_REQUIRE(nbands < 65536, GH_ERR_ARG, "lfc_apply_dH: too many bands per call");
     GH_REQUIRE(P_dev && dH_dev && out_dev, GH_ERR_ARG, "lfc_apply_dH: null argument");
     GH_REQUIRE(P_dev != out_dev, GH_ERR_ARG, "lfc_apply_dH: P and out must differ");
     hipStream_t st = (hipStream_t)stream;
