@@ -448,9 +448,21 @@ int gh_multi_dotc(int dtype, int64_t nbands, int64_t n, const void* a_dev,
                   const void* b_dev, double scale, double* out_dev,
                   double* work_dev, void* stream) {
     GH_REQUIRE(n >= 0 && nbands >= 0, GH_ERR_ARG, "multi_dotc: negative size");
-    GH_REQUIRE(nbands < 65536, GH_ERR_ARG, "multi_dotc: too many bands per call");
     if (nbands == 0) return GH_OK;
     GH_REQUIRE(a_dev && b_dev && out_dev && work_dev, GH_ERR_ARG, "multi_dotc: null argument");
+    if (nbands > kBandSlice) {
+        // one grid row per band: slices, reusing the work buffer in stream order
+        const int64_t row = n * dtype_size(dtype);
+        GH_REQUIRE(row > 0 || n == 0, GH_ERR_ARG, "multi_dotc: bad dtype");
+        for (int64_t x0 = 0; x0 < nbands; x0 += kBandSlice) {
+            const int64_t nx = std::min<int64_t>(kBandSlice, nbands - x0);
+            const int rc = gh_multi_dotc(dtype, nx, n, (const char*)a_dev + x0 * row,
+                                         (const char*)b_dev + x0 * row, scale, out_dev + x0,
+                                         work_dev, stream);
+            if (rc != GH_OK) return rc;
+        }
+        return GH_OK;
+    }
     hipStream_t st = (hipStream_t)stream;
     dim3 grid(kDotChunks, (unsigned)nbands);
     switch (dtype) {
@@ -473,11 +485,25 @@ int gh_multi_lincomb(int dtype, int64_t nbands, int64_t n, int k,
     GH_REQUIRE(n >= 0 && nbands >= 0, GH_ERR_ARG, "multi_lincomb: negative size");
     GH_REQUIRE(k >= 1 && k <= kLincombMax, GH_ERR_ARG,
                "multi_lincomb: k = %d out of range 1..%d", k, kLincombMax);
-    GH_REQUIRE(nbands < 65536, GH_ERR_ARG, "multi_lincomb: too many bands per call");
     if (nbands == 0 || n == 0) return GH_OK;  // empty arrays may carry null pointers
     GH_REQUIRE(vecs_dev && coef_dev && out_dev, GH_ERR_ARG, "multi_lincomb: null argument");
     for (int a = 0; a < k; a++)
         GH_REQUIRE(vecs_dev[a], GH_ERR_ARG, "multi_lincomb: null vector %d", a);
+    if (nbands > kBandSlice) {
+        // one grid row per band: pass larger band sets in slices
+        const int64_t row = n * dtype_size(dtype);
+        GH_REQUIRE(row > 0, GH_ERR_ARG, "multi_lincomb: bad dtype");
+        const int64_t crow = (int64_t)k * ((dtype == GH_C128 || dtype == GH_C64) ? 2 : 1);
+        const void* v[kLincombMax];
+        for (int64_t x0 = 0; x0 < nbands; x0 += kBandSlice) {
+            const int64_t nx = std::min<int64_t>(kBandSlice, nbands - x0);
+            for (int a = 0; a < k; a++) v[a] = (const char*)vecs_dev[a] + x0 * row;
+            const int rc = gh_multi_lincomb(dtype, nx, n, k, v, coef_dev + x0 * crow,
+                                            (char*)out_dev + x0 * row, stream);
+            if (rc != GH_OK) return rc;
+        }
+        return GH_OK;
+    }
     hipStream_t st = (hipStream_t)stream;
     switch (dtype) {
         case GH_F64: return lincomb_k<double, double>(k, nbands, n, vecs_dev, coef_dev, out_dev, st);
@@ -500,11 +526,25 @@ int gh_multi_gram(int dtype, int64_t nbands, int64_t n, int k,
     GH_REQUIRE(n >= 0 && nbands >= 0, GH_ERR_ARG, "multi_gram: negative size");
     GH_REQUIRE(k >= 1 && k <= kGramMax, GH_ERR_ARG, "multi_gram: k = %d out of range 1..%d",
                k, kGramMax);
-    GH_REQUIRE(nbands < 65536, GH_ERR_ARG, "multi_gram: too many bands per call");
     if (nbands == 0) return GH_OK;  // empty arrays may carry null pointers
     GH_REQUIRE(vecs_dev && out_dev && work_dev, GH_ERR_ARG, "multi_gram: null argument");
     for (int a = 0; a < k; a++)
         GH_REQUIRE(vecs_dev[a], GH_ERR_ARG, "multi_gram: null vector %d", a);
+    if (nbands > kBandSlice) {
+        // one grid row per band: slices, reusing the work buffer in stream order
+        const int64_t row = n * dtype_size(dtype);
+        GH_REQUIRE(row > 0 || n == 0, GH_ERR_ARG, "multi_gram: bad dtype");
+        const int64_t orow = (int64_t)k * k * ((dtype == GH_C128 || dtype == GH_C64) ? 2 : 1);
+        const void* v[kGramMax];
+        for (int64_t x0 = 0; x0 < nbands; x0 += kBandSlice) {
+            const int64_t nx = std::min<int64_t>(kBandSlice, nbands - x0);
+            for (int a = 0; a < k; a++) v[a] = (const char*)vecs_dev[a] + x0 * row;
+            const int rc = gh_multi_gram(dtype, nx, n, k, v, scale, out_dev + x0 * orow,
+                                         work_dev, stream);
+            if (rc != GH_OK) return rc;
+        }
+        return GH_OK;
+    }
     hipStream_t st = (hipStream_t)stream;
     switch (dtype) {
         case GH_F64: return gram_k<double>(k, nbands, n, vecs_dev, scale, 0, out_dev, work_dev, st);

@@ -108,6 +108,10 @@ inline Phases make_phases(const double* phase_cd) {
     return p;
 }
 
+// Bands per launch of the kernels that put one band on each grid row
+// (gridDim.y); the entry points pass larger band sets in slices of this size.
+constexpr int64_t kBandSlice = 32768;
+
 inline int dtype_size(int dtype) {
     switch (dtype) {
         case GH_F64: return 8;

@@ -793,9 +793,20 @@ int gh_lfc_apply_dH(const gh_lfc* h, const void* P_dev, const double* dH_dev,
     GH_REQUIRE(h, GH_ERR_ARG, "lfc_apply_dH: null handle");
     GH_REQUIRE(nbands >= 0, GH_ERR_ARG, "lfc_apply_dH: nbands < 0");
     if (nbands == 0 || h->Mmax == 0) return GH_OK;  // nothing to read or write
-    GH_REQUIRE(nbands < 65536, GH_ERR_ARG, "lfc_apply_dH: too many bands per call");
     GH_REQUIRE(P_dev && dH_dev && out_dev, GH_ERR_ARG, "lfc_apply_dH: null argument");
     GH_REQUIRE(P_dev != out_dev, GH_ERR_ARG, "lfc_apply_dH: P and out must differ");
+    if (nbands > kBandSlice) {
+        // one grid row per band: pass larger band sets in slices
+        const int64_t row = (int64_t)h->Mmax * dtype_size(h->dtype);
+        for (int64_t x0 = 0; x0 < nbands; x0 += kBandSlice) {
+            const int64_t nx = std::min<int64_t>(kBandSlice, nbands - x0);
+            const int rc = gh_lfc_apply_dH(h, (const char*)P_dev + x0 * row, dH_dev,
+                                           eps_dev ? eps_dev + x0 : nullptr, dS_dev,
+                                           (char*)out_dev + x0 * row, nx, stream);
+            if (rc != GH_OK) return rc;
+        }
+        return GH_OK;
+    }
     hipStream_t st = (hipStream_t)stream;
     dim3 grid((h->Mmax + 127) / 128, (unsigned)nbands);
 #define DH_ARGS(V) (const V*)P_dev, dH_dev, h->dH_off, eps_dev, dS_dev, h->M_ptr, \

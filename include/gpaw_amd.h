@@ -19,8 +19,8 @@
  * the leading axis allow) returns GH_OK without touching memory; its array
  * pointers may then be NULL, as empty device arrays' pointers are.
  * gh_lfc_apply_dH, gh_multi_dotc, gh_multi_gram and gh_multi_lincomb launch
- * one grid row per band and take at most 65535 bands per call (GH_ERR_ARG
- * beyond); larger band sets are passed in slices.
+ * one grid row per band; they split band sets above 32768 into slices of
+ * that size themselves (one launch sequence per slice, in stream order).
  */
 #ifndef GPAW_AMD_H
 #define GPAW_AMD_H
