@@ -333,7 +333,15 @@ restrict_kernel(const V* __restrict__ in, V* __restrict__ out, TGeom g,
         kz[k] = pp - jl[k] * m2;
     }
     constexpr int QL = 2 * K - 1;
+    // register queue of z- and y-filtered planes per point; every access
+    // uses a compile-time slot (the new planes land in the last slots and
+    // the queue shifts), so it stays in registers
     V q[PPT][QL];
+#pragma unroll
+    for (int k = 0; k < PPT; k++)
+#pragma unroll
+        for (int s = 0; s < QL; s++) q[k][s] = vzero<V>();
+    struct ZRow { V v[PPT]; };
     const int base = 2 * I0 - g.plo[0];  // first input plane of output I0
 
     // plane ix is in registers on entry (loaded during the previous
@@ -343,7 +351,7 @@ restrict_kernel(const V* __restrict__ in, V* __restrict__ out, TGeom g,
         if constexpr (VEC > 1) stage_load_v<V, S, VEC>(inb, ix, iy0, nrows, pre, g, ph, wz);
         else stage_load<V, S>(inb, ix, iy0, nrows, pre, g, ph, wz);
     };
-    auto process_plane = [&](int ix, int slot) {
+    auto process_plane = [&](int ix) -> ZRow {
         if constexpr (VEC > 1) stage_store_v<V, S, VEC>(inb, ix, iy0, nrows, pre, A, g, ph, wz);
         else stage_store<V, S>(inb, ix, iy0, nrows, pre, A, g, ph, wz);
         __syncthreads();
@@ -366,22 +374,40 @@ restrict_kernel(const V* __restrict__ in, V* __restrict__ out, TGeom g,
         }
         __syncthreads();
         // z-filter at the own points
+        ZRow z;
 #pragma unroll
         for (int k = 0; k < PPT; k++) {
+            z.v[k] = vzero<V>();
             if (!act[k]) continue;
             const int c = 2 * kz[k] + (K - 1) - g.plo[2];
-            q[k][slot] = restrict_tap<V, S, K>(B + jl[k] * g.zw + g.zlo, c, g, ph);
+            z.v[k] = restrict_tap<V, S, K>(B + jl[k] * g.zw + g.zlo, c, g, ph);
         }
+        return z;
     };
 
-    // prime the first 2K-3 planes
+    // prime the first 2K-3 planes: they end in slots 2 .. QL-1
     sload(base);
+    for (int s = 0; s < QL - 2; s++) {
+        const ZRow z = process_plane(base + s);
 #pragma unroll
-    for (int s = 0; s < QL - 2; s++) process_plane(base + s, s);
+        for (int k = 0; k < PPT; k++) {
+#pragma unroll
+            for (int t = 0; t < QL - 1; t++) q[k][t] = q[k][t + 1];
+            q[k][QL - 1] = z.v[k];
+        }
+    }
     for (int I = I0; I < I1; I++) {
+#pragma unroll
+        for (int k = 0; k < PPT; k++)
+#pragma unroll
+            for (int s = 0; s < QL - 2; s++) q[k][s] = q[k][s + 2];
         int p0 = base + 2 * (I - I0) + QL - 2;
-        process_plane(p0, QL - 2);
-        process_plane(p0 + 1, QL - 1);
+        const ZRow z0 = process_plane(p0);
+#pragma unroll
+        for (int k = 0; k < PPT; k++) q[k][QL - 2] = z0.v[k];
+        const ZRow z1 = process_plane(p0 + 1);
+#pragma unroll
+        for (int k = 0; k < PPT; k++) q[k][QL - 1] = z1.v[k];
 #pragma unroll
         for (int k = 0; k < PPT; k++) {
             if (!act[k]) continue;
@@ -395,10 +421,6 @@ restrict_kernel(const V* __restrict__ in, V* __restrict__ out, TGeom g,
             outb[((size_t)I * g.m[1] + J0 + jl[k]) * m2 + kz[k]] =
                 vscale(vscale(acc, (S)0.5), alpha);
         }
-#pragma unroll
-        for (int k = 0; k < PPT; k++)
-#pragma unroll
-            for (int s = 0; s < QL - 2; s++) q[k][s] = q[k][s + 2];
     }
 }
 
