@@ -333,14 +333,19 @@ restrict_kernel(const V* __restrict__ in, V* __restrict__ out, TGeom g,
         kz[k] = pp - jl[k] * m2;
     }
     constexpr int QL = 2 * K - 1;
-    // register queue of z- and y-filtered planes per point; every access
-    // uses a compile-time slot (the new planes land in the last slots and
-    // the queue shifts), so it stays in registers
+    // register queue of z- and y-filtered planes per point.  For complex64
+    // every access uses a compile-time slot (the new planes land in the last
+    // slots and the queue shifts): priming it slot by slot put that queue in
+    // scratch memory and doubled the kernel's time.  The other types keep
+    // the slot-by-slot priming, measured faster for them.
+    constexpr bool kShiftIn = std::is_same<V, cplx<float>>::value;
     V q[PPT][QL];
+    if constexpr (kShiftIn) {
 #pragma unroll
-    for (int k = 0; k < PPT; k++)
+        for (int k = 0; k < PPT; k++)
 #pragma unroll
-        for (int s = 0; s < QL; s++) q[k][s] = vzero<V>();
+            for (int s = 0; s < QL; s++) q[k][s] = vzero<V>();
+    }
     struct ZRow { V v[PPT]; };
     const int base = 2 * I0 - g.plo[0];  // first input plane of output I0
 
@@ -385,22 +390,34 @@ restrict_kernel(const V* __restrict__ in, V* __restrict__ out, TGeom g,
         return z;
     };
 
-    // prime the first 2K-3 planes: they end in slots 2 .. QL-1
+    // prime the first 2K-3 planes: slots 0 .. QL-3 (shift-in form: they
+    // end in slots 2 .. QL-1 and move down at the top of each output plane)
     sload(base);
-    for (int s = 0; s < QL - 2; s++) {
-        const ZRow z = process_plane(base + s);
+    if constexpr (kShiftIn) {
+        for (int s = 0; s < QL - 2; s++) {
+            const ZRow z = process_plane(base + s);
 #pragma unroll
-        for (int k = 0; k < PPT; k++) {
+            for (int k = 0; k < PPT; k++) {
 #pragma unroll
-            for (int t = 0; t < QL - 1; t++) q[k][t] = q[k][t + 1];
-            q[k][QL - 1] = z.v[k];
+                for (int t = 0; t < QL - 1; t++) q[k][t] = q[k][t + 1];
+                q[k][QL - 1] = z.v[k];
+            }
+        }
+    } else {
+#pragma unroll
+        for (int s = 0; s < QL - 2; s++) {
+            const ZRow z = process_plane(base + s);
+#pragma unroll
+            for (int k = 0; k < PPT; k++) q[k][s] = z.v[k];
         }
     }
     for (int I = I0; I < I1; I++) {
+        if constexpr (kShiftIn) {
 #pragma unroll
-        for (int k = 0; k < PPT; k++)
+            for (int k = 0; k < PPT; k++)
 #pragma unroll
-            for (int s = 0; s < QL - 2; s++) q[k][s] = q[k][s + 2];
+                for (int s = 0; s < QL - 2; s++) q[k][s] = q[k][s + 2];
+        }
         int p0 = base + 2 * (I - I0) + QL - 2;
         const ZRow z0 = process_plane(p0);
 #pragma unroll
@@ -420,6 +437,12 @@ restrict_kernel(const V* __restrict__ in, V* __restrict__ out, TGeom g,
             }
             outb[((size_t)I * g.m[1] + J0 + jl[k]) * m2 + kz[k]] =
                 vscale(vscale(acc, (S)0.5), alpha);
+        }
+        if constexpr (!kShiftIn) {
+#pragma unroll
+            for (int k = 0; k < PPT; k++)
+#pragma unroll
+                for (int s = 0; s < QL - 2; s++) q[k][s] = q[k][s + 2];
         }
     }
 }
@@ -479,19 +502,11 @@ interpolate_kernel(const V* __restrict__ in, V* __restrict__ out, TGeom g,
         jl[k] = pp / m2;
         kz[k] = pp - jl[k] * m2;
     }
-    // coarse planes c - K + 1 .. c after the y/z filters; every access uses
-    // a compile-time slot (the new plane lands in the last slot and the
-    // queue shifts), so it stays in registers
-    V q[PPT][K];
-#pragma unroll
-    for (int k = 0; k < PPT; k++)
-#pragma unroll
-        for (int s = 0; s < K; s++) q[k][s] = vzero<V>();
-    struct ZRow { V v[PPT]; };
+    V q[PPT][K];  // coarse planes c - K + 1 .. c after the y/z filters
 
     // plane ix is in registers on entry; ix + 1 is loaded during its filters
     StageRegs<V> pre;
-    auto process_plane = [&](int ix) -> ZRow {
+    auto process_plane = [&](int ix, int slot) {
         stage_store<V, S>(inb, ix, cy0, nrows, pre, A, g, ph, wz);
         __syncthreads();
         stage_load<V, S>(inb, ix + 1, cy0, nrows, pre, g, ph, wz);
@@ -502,10 +517,8 @@ interpolate_kernel(const V* __restrict__ in, V* __restrict__ out, TGeom g,
                                                  g.plo[2], g, ph);
         }
         __syncthreads();
-        ZRow z;
 #pragma unroll
         for (int k = 0; k < PPT; k++) {
-            z.v[k] = vzero<V>();
             if (!act[k]) continue;
             int J = J0 + jl[k];
             int qq = J + g.skip[1];
@@ -521,9 +534,8 @@ interpolate_kernel(const V* __restrict__ in, V* __restrict__ out, TGeom g,
                                vadd(B[(c - t) * m2 + kz[k]], B[(c + 1 + t) * m2 + kz[k]]),
                                v);
             }
-            z.v[k] = v;
+            q[k][slot] = v;
         }
-        return z;
     };
 
     // fine plane o -> coarse centre c(o) = ((o + skip) >> 1) + nn - 1 - plo;
@@ -532,24 +544,10 @@ interpolate_kernel(const V* __restrict__ in, V* __restrict__ out, TGeom g,
     const int i_last = (O1 - 1 + g.skip[0]) >> 1;
     const int cbase = nn - 1 - g.plo[0];
     stage_load<V, S>(inb, i_first + cbase - nn + 1, cy0, nrows, pre, g, ph, wz);
-    // prime K - 1 planes: they end in slots 1 .. K - 1
-    for (int s = 0; s < K - 1; s++) {
-        const ZRow z = process_plane(i_first + cbase - nn + 1 + s);
 #pragma unroll
-        for (int k = 0; k < PPT; k++) {
-#pragma unroll
-            for (int t = 0; t < K - 1; t++) q[k][t] = q[k][t + 1];
-            q[k][K - 1] = z.v[k];
-        }
-    }
+    for (int s = 0; s < K - 1; s++) process_plane(i_first + cbase - nn + 1 + s, s);
     for (int i = i_first; i <= i_last; i++) {
-#pragma unroll
-        for (int k = 0; k < PPT; k++)
-#pragma unroll
-            for (int s = 0; s < K - 1; s++) q[k][s] = q[k][s + 1];
-        const ZRow z = process_plane(i + cbase + nn);
-#pragma unroll
-        for (int k = 0; k < PPT; k++) q[k][K - 1] = z.v[k];
+        process_plane(i + cbase + nn, K - 1);
         // queue slot s holds coarse plane (c - nn + 1 + s), c = i + cbase
         const int oe = 2 * i - g.skip[0];
 #pragma unroll
@@ -571,6 +569,10 @@ interpolate_kernel(const V* __restrict__ in, V* __restrict__ out, TGeom g,
                 outb[((size_t)o * g.m[1] + J0 + jl[k]) * m2 + kz[k]] = vscale(v, alpha);
             }
         }
+#pragma unroll
+        for (int k = 0; k < PPT; k++)
+#pragma unroll
+            for (int s = 0; s < K - 1; s++) q[k][s] = q[k][s + 1];
     }
 }
 
